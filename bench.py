@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the plain SPH step at 1,048,576 particles on one MI355X.
+
+Workload: dam break, 128 x 128 x 64 lattice at 0.0078 m spacing (jittered, seed 12345)
+in a 2 x 1 x 1 m box, h = 0.0156 m, linear EOS k = 2000, viscosity 0.5, 2^21 hash
+buckets, step(1/600 s) = 8 substeps. A "step" is one FluidSolver::step call over the
+whole particle set; inputs are resident in HBM before the timed region starts.
+
+Printed: one JSON line with the throughput, the roofline of the dominant kernel
+(achieved algorithmic bytes per launch / its mean HIP-event duration, against the
+8 TB/s HBM peak) and the CPU baseline (the oracle's FP32 restatement of the reference
+step on this host's cores over a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s at 1M particles, plain vs surrogate, at ≤1% density error"
+HBM_PEAK_GBS = 8000.0
+DT = 1.0 / 600.0
+
+# Algorithmic bytes per particle-substep (FP32, counting sort with one reorder, neighbour
+# reads served on chip and not counted): keys 16, rank/scatter 8, reorder 52, density 20,
+# force/integrate 56 (read pos, vel, density, pressure 32 + write pos, vel 24), plus the
+# per-bucket counts/starts 8 C / N.
+BYTES_FORCE = 56.0
+BYTES_SUBSTEP_FIXED = 152.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=1 << 21)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
+    """The oracle's FP32 restatement of FluidSolver::step on this host (test
+    infrastructure, used here only as the reported baseline)."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    s = O.OracleSolver(32)
+    s.load_state(pos, vel)
+    substeps = 8
+    dt = DT / substeps
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        s.substep(p, dt, threads)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= 64:
+            break
+    rate_sub = w.count * done / el
+    return {
+        "value": rate_sub / substeps,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} substeps (= {done / substeps:.3g} step(1/600) calls) of the same 1M-particle "
+                  f"dam-break state, FP32 reference-semantics restatement, {threads} threads, {el:.1f} s",
+        "particle_substeps_per_s": rate_sub,
+    }
+
+
+def load_traffic():
+    f = ROOT / "profiles" / "traffic_force_integrate_dam_break_1m.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from computer_graphics_lab_00_amd import fluid
+
+    w = fluid.m1(args.cells)
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    n = w.count
+    substeps = fluid.fluid_substeps(p, DT)
+    print(f"[bench] workload {w.name}: {n} particles, {p.cellCount} buckets, {substeps} substeps per step",
+          file=sys.stderr, flush=True)
+
+    gpu = fluid.FluidSolver(n, max(p.cellCount, 1024))
+    gpu.load_state(pos, vel)
+    for _ in range(args.warmup):
+        gpu.step(p, DT)
+    gpu.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    gpu.profile_enable(True)
+    gpu.profile_read()   # reset
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gpu.step(p, DT)
+    gpu.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = gpu.profile_read()
+    gpu.profile_enable(False)
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    value = n * args.steps / elapsed
+    ms_per_step = elapsed * 1e3 / args.steps
+
+    # dominant kernel: the largest total event time
+    kernel_ms = {k: v[0] for k, v in prof.items()}
+    dom = max(kernel_ms, key=kernel_ms.get)
+    launches = prof[dom][1]
+    avg_ms = prof[dom][0] / max(launches, 1)
+    per_unit = BYTES_FORCE if dom == "force_integrate" else None
+    achieved = (per_unit * n / (avg_ms * 1e-3)) / 1e9 if per_unit else None
+    substep_ms = sum(kernel_ms.values()) / max(prof["force_integrate"][1], 1)
+    b_alg = BYTES_SUBSTEP_FIXED + 8.0 * p.cellCount / n
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+        "traffic": load_traffic(),
+        "bytes_per_unit": per_unit,
+        "units_per_launch": n,
+        "avg_launch_ms": avg_ms,
+        "substep": {"bytes_per_particle": b_alg, "ms": substep_ms,
+                    "achieved": b_alg * n / (substep_ms * 1e-3) / 1e9,
+                    "frac": b_alg * n / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "kernels_ms_per_substep": {k: v[0] / max(v[1], 1) for k, v in prof.items()},
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, p, pos, vel, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "particles": n, "lattice": list(w.lattice), "spacing": w.spacing,
+                   "smoothing_radius": w.smoothing, "cell_count": p.cellCount, "substeps_per_step": substeps,
+                   "dt": DT, "eos": "linear k=2000", "mode": "reference (32-slot cap, duplicate visits)",
+                   "particle_substeps_per_s": value * substeps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    gpu.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,373 @@
+// sph_forces.hip — density pass and force + integrate + boundary pass for gfx950.
+//
+// Replace the two parallel loops of FluidSolver::substep (src/physics/fluid_sph.cpp:197-305)
+// and shaders/fluid_density.comp / fluid_integrate.comp. One lane owns one particle of
+// the bucket-sorted state and walks its 27 neighbour buckets in the reference order
+// (dz outer, dy, dx inner), each bucket's slots in ascending particle index, so every
+// floating-point sum is accumulated in exactly the reference's order. Built with
+// -ffp-contract=off and correctly rounded division / square root, so each operation
+// rounds as the reference's does.
+//
+// Consecutive lanes hold particles of the same bucket (the sort put them together), so
+// a wave's candidate loads hit the same few cache lines: neighbour reads are served
+// from L1/L2 and HBM sees each particle's state about once per pass.
+#include "sph_device.h"
+
+namespace sphdev {
+
+namespace {
+
+__device__ inline uint32_t hash_cell(int32_t cx, int32_t cy, int32_t cz, uint32_t mask) {
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & mask;
+}
+
+__device__ inline int32_t cell_coord(float x, float h) { return (int32_t)floorf(x / h); }
+
+// Whether the bucket of neighbour offset `o` (0..26) already appeared at an earlier offset.
+__device__ inline bool seen_before(int32_t cx, int32_t cy, int32_t cz, uint32_t o, uint32_t b, uint32_t mask) {
+    for (uint32_t e = 0; e < o; ++e) {
+        int32_t dx = (int32_t)(e % 3u) - 1, dy = (int32_t)((e / 3u) % 3u) - 1, dz = (int32_t)(e / 9u) - 1;
+        if (hash_cell(cx + dx, cy + dy, cz + dz, mask) == b) return true;
+    }
+    return false;
+}
+
+__device__ inline float eos_pressure(const StepConsts& k, float density) {
+    float p;
+    if (k.eos == SPH_EOS_TAIT) {
+        float x = density / k.restDensity;
+        float x2 = x * x;
+        float x4 = x2 * x2;
+        float x7 = (x4 * x2) * x;
+        p = k.taitB * (x7 - 1.0f);
+    } else {
+        p = k.stiffness * (density - k.restDensity);   // src/physics/fluid_sph.cpp:220
+    }
+    return p < 0.0f ? 0.0f : p;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_density(StepConsts k, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ starts) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= k.n) return;
+    // Only xyz of any slot is read here; .w of this lane's own slot is written below
+    // (the reference writes positions[i].w in place the same way, fluid_sph.cpp:197-199).
+    const float* self = reinterpret_cast<const float*>(pos + t);
+    const float px = self[0], py = self[1], pz = self[2];
+    const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
+    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+    const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
+    float density = 0.0f;
+    uint32_t o = 0;
+    for (int32_t dz = -1; dz <= 1; ++dz) {
+        for (int32_t dy = -1; dy <= 1; ++dy) {
+            for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
+                const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
+                if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
+                const uint32_t lo = starts[b];
+                uint32_t cnt = starts[b + 1] - lo;
+                if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+                const float* q = reinterpret_cast<const float*>(pos + lo);
+                for (uint32_t j = 0; j < cnt; ++j, q += 4) {
+                    const float ddx = px - q[0], ddy = py - q[1], ddz = pz - q[2];
+                    const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    // poly6 is 0 at r2 >= h2 and density + mass * 0 == density: skip.
+                    if (r2 < k.h2) {
+                        const float d = k.h2 - r2;
+                        density += k.mass * (((k.cPoly6 * d) * d) * d);
+                    }
+                }
+            }
+        }
+    }
+    density = density < 1e-6f ? 1e-6f : density;
+    reinterpret_cast<float*>(pos + t)[3] = density;
+    reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
+}
+
+// ---- boundary pieces (src/physics/force_field.h:31-70, collider_shapes.h:33-131) ----
+
+struct Surface {
+    float p[3];
+    float n[3];
+    float dist;
+};
+
+__device__ inline void sphere_local(float radius, const float p[3], Surface& s) {
+    float d = sqrtf((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]);
+    if (d > 1.0e-6f) {
+        s.n[0] = p[0] / d; s.n[1] = p[1] / d; s.n[2] = p[2] / d;
+    } else {
+        s.n[0] = 0.0f; s.n[1] = 1.0f; s.n[2] = 0.0f;
+    }
+    s.p[0] = s.n[0] * radius; s.p[1] = s.n[1] * radius; s.p[2] = s.n[2] * radius;
+    s.dist = d - radius;
+}
+
+__device__ inline float clampf_ref(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+__device__ void closest_local(uint32_t shape, float radius, const float* half, const float p[3], Surface& s) {
+    switch (shape) {
+    case SPH_SHAPE_SPHERE: sphere_local(radius, p, s); return;
+    case SPH_SHAPE_BOX: {
+        float cl[3], dl[3];
+        for (int a = 0; a < 3; ++a) {
+            float lo = -half[a];
+            float mx = p[a] < lo ? lo : p[a];
+            cl[a] = half[a] < mx ? half[a] : mx;
+            dl[a] = p[a] - cl[a];
+        }
+        float dsq = (dl[0] * dl[0] + dl[1] * dl[1]) + dl[2] * dl[2];
+        if (dsq > 1.0e-10f) {
+            float dist = sqrtf(dsq);
+            s.n[0] = dl[0] / dist; s.n[1] = dl[1] / dist; s.n[2] = dl[2] / dist;
+            s.p[0] = cl[0]; s.p[1] = cl[1]; s.p[2] = cl[2];
+            s.dist = dist;
+            return;
+        }
+        float gap[3];
+        for (int a = 0; a < 3; ++a) gap[a] = half[a] - (p[a] < 0.0f ? -p[a] : p[a]);
+        int axis = gap[0] < gap[1] ? (gap[0] < gap[2] ? 0 : 2) : (gap[1] < gap[2] ? 1 : 2);
+        s.n[0] = s.n[1] = s.n[2] = 0.0f;
+        s.n[axis] = p[axis] >= 0.0f ? 1.0f : -1.0f;
+        s.p[0] = p[0]; s.p[1] = p[1]; s.p[2] = p[2];
+        s.p[axis] = s.n[axis] * half[axis];
+        s.dist = -gap[axis];
+        return;
+    }
+    case SPH_SHAPE_PLANE:
+        s.p[0] = p[0]; s.p[1] = 0.0f; s.p[2] = p[2];
+        s.n[0] = 0.0f; s.n[1] = 1.0f; s.n[2] = 0.0f;
+        s.dist = p[1];
+        return;
+    case SPH_SHAPE_CYLINDER: {
+        float hh = half[1];
+        float d = sqrtf((p[0] * p[0] + 0.0f * 0.0f) + p[2] * p[2]);
+        float rd[3];
+        if (d > 1.0e-6f) {
+            rd[0] = p[0] / d; rd[1] = 0.0f / d; rd[2] = p[2] / d;
+        } else {
+            rd[0] = 1.0f; rd[1] = 0.0f; rd[2] = 0.0f;
+        }
+        float ay = p[1] < 0.0f ? -p[1] : p[1];
+        if (ay <= hh && d <= radius) {
+            float gapY = hh - ay, gapR = radius - d;
+            if (gapY < gapR) {
+                float side = p[1] >= 0.0f ? 1.0f : -1.0f;
+                s.n[0] = 0.0f; s.n[1] = side; s.n[2] = 0.0f;
+                s.p[0] = p[0]; s.p[1] = side * hh; s.p[2] = p[2];
+                s.dist = -gapY;
+            } else {
+                s.n[0] = rd[0]; s.n[1] = rd[1]; s.n[2] = rd[2];
+                s.p[0] = rd[0] * radius + 0.0f; s.p[1] = rd[1] * radius + p[1]; s.p[2] = rd[2] * radius + 0.0f;
+                s.dist = -gapR;
+            }
+            return;
+        }
+        float m = d < radius ? d : radius;
+        float cl[3] = {rd[0] * m, rd[1] * m, rd[2] * m};
+        cl[1] = clampf_ref(p[1], -hh, hh);
+        float dl[3] = {p[0] - cl[0], p[1] - cl[1], p[2] - cl[2]};
+        float dist = sqrtf((dl[0] * dl[0] + dl[1] * dl[1]) + dl[2] * dl[2]);
+        s.p[0] = cl[0]; s.p[1] = cl[1]; s.p[2] = cl[2];
+        if (dist > 1.0e-6f) {
+            s.n[0] = dl[0] / dist; s.n[1] = dl[1] / dist; s.n[2] = dl[2] / dist;
+        } else {
+            s.n[0] = 0.0f; s.n[1] = 1.0f; s.n[2] = 0.0f;
+        }
+        s.dist = dist;
+        return;
+    }
+    case SPH_SHAPE_CAPSULE: {
+        float hh = half[1];
+        float seg = clampf_ref(p[1], -hh, hh);
+        float q[3] = {p[0] - 0.0f, p[1] - seg, p[2] - 0.0f};
+        sphere_local(radius, q, s);
+        s.p[0] += 0.0f; s.p[1] += seg; s.p[2] += 0.0f;
+        return;
+    }
+    default: {
+        sphere_local(0.0f, p, s);
+        return;
+    }
+    }
+}
+
+__device__ void collide(const SphCollider& c, float radius, float restitution, float pp[3], float vv[3]) {
+    float loc[3];
+    for (int r = 0; r < 3; ++r) {
+        float a0 = c.inverseWorld[0 + r] * pp[0] + c.inverseWorld[4 + r] * pp[1];
+        float a1 = c.inverseWorld[8 + r] * pp[2] + c.inverseWorld[12 + r] * 1.0f;
+        loc[r] = a0 + a1;
+    }
+    Surface s;
+    closest_local(c.shape, c.radius, c.halfExtents, loc, s);
+    if (!(s.dist < radius)) return;
+    float nw[3];
+    for (int r = 0; r < 3; ++r) nw[r] = (c.world[0 + r] * s.n[0] + c.world[4 + r] * s.n[1]) + c.world[8 + r] * s.n[2];
+    float inv = 1.0f / sqrtf((nw[0] * nw[0] + nw[1] * nw[1]) + nw[2] * nw[2]);
+    float n[3] = {nw[0] * inv, nw[1] * inv, nw[2] * inv};
+    for (int r = 0; r < 3; ++r) {
+        float a0 = c.world[0 + r] * s.p[0] + c.world[4 + r] * s.p[1];
+        float a1 = c.world[8 + r] * s.p[2] + c.world[12 + r] * 1.0f;
+        pp[r] = (a0 + a1) + n[r] * radius;
+    }
+    float along = (vv[0] * n[0] + vv[1] * n[1]) + vv[2] * n[2];
+    if (along < 0.0f) {
+        float sc = (1.0f + restitution) * along;
+        vv[0] -= sc * n[0]; vv[1] -= sc * n[1]; vv[2] -= sc * n[2];
+    }
+}
+
+__device__ void field_one(const SphForceField& f, const float p[3], float out[3]) {
+    float off[3] = {p[0] - f.position[0], p[1] - f.position[1], p[2] - f.position[2]};
+    float distance = sqrtf((off[0] * off[0] + off[1] * off[1]) + off[2] * off[2]);
+    float weight = 1.0f;
+    if (f.radius > 0.0f) {
+        float t = clampf_ref(1.0f - distance / f.radius, 0.0f, 1.0f);
+        weight = t <= 0.0f ? 0.0f : powf(t, f.falloff);
+    }
+    out[0] = out[1] = out[2] = 0.0f;
+    if (weight <= 0.0f) return;
+    float sw = f.strength * weight;
+    if (f.type == SPH_FIELD_WIND) {
+        out[0] = f.axis[0] * sw; out[1] = f.axis[1] * sw; out[2] = f.axis[2] * sw;
+    } else if (f.type == SPH_FIELD_VORTEX) {
+        float along = (off[0] * f.axis[0] + off[1] * f.axis[1]) + off[2] * f.axis[2];
+        float rad[3] = {off[0] - f.axis[0] * along, off[1] - f.axis[1] * along, off[2] - f.axis[2] * along};
+        float r = sqrtf((rad[0] * rad[0] + rad[1] * rad[1]) + rad[2] * rad[2]);
+        if (r < 1.0e-4f) return;
+        float u[3] = {rad[0] / r, rad[1] / r, rad[2] / r};
+        out[0] = (f.axis[1] * u[2] - u[1] * f.axis[2]) * sw;
+        out[1] = (f.axis[2] * u[0] - u[2] * f.axis[0]) * sw;
+        out[2] = (f.axis[0] * u[1] - u[0] * f.axis[1]) * sw;
+    } else if (f.type == SPH_FIELD_POINT) {
+        if (distance < 1.0e-4f) return;
+        out[0] = (-off[0] / distance) * sw;
+        out[1] = (-off[1] / distance) * sw;
+        out[2] = (-off[2] / distance) * sw;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_force(StepConsts k, BoundaryConsts bnd, const float4* __restrict__ pos, const float4* __restrict__ vel,
+            const uint32_t* __restrict__ starts, float4* __restrict__ posOut, float4* __restrict__ velOut,
+            float4* __restrict__ forceOut) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= k.n) return;
+    const float4 self = pos[t];
+    const float4 selfVel = vel[t];
+    float px = self.x, py = self.y, pz = self.z;
+    const float density = self.w < 1e-6f ? 1e-6f : self.w;
+    float vx = selfVel.x, vy = selfVel.y, vz = selfVel.z;
+    const float pressure = selfVel.w;
+    const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
+    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+    const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
+
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    uint32_t o = 0;
+    for (int32_t dz = -1; dz <= 1; ++dz) {
+        for (int32_t dy = -1; dy <= 1; ++dy) {
+            for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
+                const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
+                if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
+                const uint32_t lo = starts[b];
+                uint32_t cnt = starts[b + 1] - lo;
+                if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const uint32_t u = lo + j;
+                    if (u == t) continue;
+                    const float4 q = pos[u];
+                    const float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
+                    const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    // r2 >= r2Cut <=> sqrtf(r2) >= h (both correctly rounded, monotone):
+                    // the reference's `distance >= h` test without the square root.
+                    if (r2 >= k.r2Cut) continue;
+                    const float distance = sqrtf(r2);
+                    if (distance >= k.h || distance <= 1e-6f) continue;
+                    const float otherDensity = q.w < 1e-6f ? 1e-6f : q.w;
+                    const float4 ov = vel[u];
+                    // spikyGradient (fluid_sph.cpp:33-39): ((c d) d) (delta / length)
+                    const float d = k.h - distance;
+                    const float g = (k.cSpiky * d) * d;
+                    const float gx = g * (ddx / distance), gy = g * (ddy / distance), gz = g * (ddz / distance);
+                    // -mass (p_i + p_j) / (2 rho_j) * grad (fluid_sph.cpp:251-252)
+                    const float sp = (k.negMass * (pressure + ov.w)) / (2.0f * otherDensity);
+                    fx += sp * gx;
+                    fy += sp * gy;
+                    fz += sp * gz;
+                    // viscosity mass (v_j - v_i) / rho_j * laplacian (fluid_sph.cpp:253-254)
+                    const float lap = k.cVisc * (k.h - distance);
+                    fx += ((k.viscMass * (ov.x - vx)) / otherDensity) * lap;
+                    fy += ((k.viscMass * (ov.y - vy)) / otherDensity) * lap;
+                    fz += ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
+                }
+            }
+        }
+    }
+    if (forceOut) forceOut[t] = make_float4(fx, fy, fz, 0.0f);
+
+    // acceleration = force / density + gravity + fields (fluid_sph.cpp:260-261)
+    float fa[3] = {0.0f, 0.0f, 0.0f};
+    const float here[3] = {px, py, pz};
+    for (uint32_t f = 0; f < k.fieldCount; ++f) {
+        float a[3];
+        field_one(bnd.fields[f], here, a);
+        fa[0] += a[0]; fa[1] += a[1]; fa[2] += a[2];
+    }
+    const float ax = (fx / density + k.gravity[0]) + fa[0];
+    const float ay = (fy / density + k.gravity[1]) + fa[1];
+    const float az = (fz / density + k.gravity[2]) + fa[2];
+    vx += ax * k.dt;
+    vy += ay * k.dt;
+    vz += az * k.dt;
+    // speed clamp (fluid_sph.cpp:264-268)
+    const float speed = sqrtf((vx * vx + vy * vy) + vz * vz);
+    if (speed > k.maxSpeed) {
+        const float sc = k.maxSpeed / speed;
+        vx *= sc;
+        vy *= sc;
+        vz *= sc;
+    }
+    px += vx * k.dt;
+    py += vy * k.dt;
+    pz += vz * k.dt;
+    // container walls (fluid_sph.cpp:271-285)
+    float pp[3] = {px, py, pz};
+    float vv[3] = {vx, vy, vz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (pp[a] < k.low[a]) {
+            pp[a] = k.low[a];
+            if (vv[a] < 0.0f) vv[a] = -vv[a] * k.restitution;
+        } else if (pp[a] > k.high[a]) {
+            pp[a] = k.high[a];
+            if (vv[a] > 0.0f) vv[a] = -vv[a] * k.restitution;
+        }
+    }
+    // colliders (fluid_sph.cpp:289-300)
+    for (uint32_t c = 0; c < k.colliderCount; ++c) collide(bnd.colliders[c], k.radius, k.restitution, pp, vv);
+
+    posOut[t] = make_float4(pp[0], pp[1], pp[2], density);
+    velOut[t] = make_float4(vv[0], vv[1], vv[2], pressure);
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* starts, hipStream_t s) {
+    if (k.n == 0) return hipSuccess;
+    k_density<<<blocks_for(k.n), kBlock, 0, s>>>(k, pos, vel, starts);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const StepConsts& k, const BoundaryConsts& b, const float4* pos, const float4* vel,
+                        const uint32_t* starts, float4* posOut, float4* velOut, float4* forceOut, hipStream_t s) {
+    if (k.n == 0) return hipSuccess;
+    k_force<<<blocks_for(k.n), kBlock, 0, s>>>(k, b, pos, vel, starts, posOut, velOut, forceOut);
+    return hipGetLastError();
+}
+
+}  // namespace sphdev

@@ -1,0 +1,266 @@
+/*
+ * sph_c_api.h — C ABI of the MI355X-native SPH fluid step.
+ *
+ * This is the drop-in boundary for the reference's `src/physics` fluid interface
+ * (namespace physics, src/physics/fluid_sph.h:42-127). Every entry point below names
+ * the reference function or member it replaces. Plain C types only: opaque handles,
+ * caller-owned host buffers, status-code returns, no exceptions, no global state
+ * other than the last-error string of the calling thread.
+ *
+ * Layout conventions shared with the reference:
+ *   - particle arrays crossing the ABI are AoS vec4 in particle-index order:
+ *     positions (x, y, z, density), velocities (vx, vy, vz, pressure)
+ *     — the layout of FluidSolver::particles() (src/physics/fluid_sph.h:108-123);
+ *   - 4x4 matrices are column-major float[16], the glm::mat4 memory layout.
+ *
+ * Inside the library the state lives in HBM in bucket-sorted order with carried
+ * particle ids; reads un-permute it back to index order.
+ */
+#ifndef SPH_C_API_H
+#define SPH_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/physics/fluid_sph.h:22-23 (FLUID_MAX_COLLIDERS, FLUID_CELL_CAPACITY),
+ * src/physics/force_field.h:17 (MAX_FORCE_FIELDS). */
+#define SPH_MAX_COLLIDERS 8u
+#define SPH_MAX_FORCE_FIELDS 8u
+#define SPH_CELL_CAPACITY 32u
+/* src/physics/fluid_sph.h:76 (FLUID_MAX_FRAME_STEP). */
+#define SPH_MAX_FRAME_STEP (1.0f / 30.0f)
+
+/* Status codes. */
+typedef enum SphStatus {
+    SPH_OK = 0,
+    SPH_ERR_INVALID_ARGUMENT = 1,
+    SPH_ERR_NO_DEVICE = 2,
+    SPH_ERR_OUT_OF_MEMORY = 3,
+    SPH_ERR_DEVICE = 4,          /* a HIP runtime call failed */
+    SPH_ERR_CAPACITY = 5,        /* count or cellCount above what the handle was created for */
+    SPH_ERR_NOT_POWER_OF_TWO = 6,
+    SPH_ERR_STATE = 7            /* call not valid in the current state (e.g. read before emit) */
+} SphStatus;
+
+/* scene::ColliderShape (src/scene/scene.h:119-131). */
+typedef enum SphColliderShape {
+    SPH_SHAPE_SPHERE = 0,
+    SPH_SHAPE_BOX = 1,
+    SPH_SHAPE_PLANE = 2,
+    SPH_SHAPE_CYLINDER = 3,
+    SPH_SHAPE_CAPSULE = 4,
+    SPH_SHAPE_MESH = 5
+} SphColliderShape;
+
+/* scene::ForceFieldType (src/scene/scene.h:287-291). */
+typedef enum SphForceFieldType {
+    SPH_FIELD_WIND = 0,
+    SPH_FIELD_VORTEX = 1,
+    SPH_FIELD_POINT = 2
+} SphForceFieldType;
+
+/* physics::FluidCollider (src/physics/fluid_sph.h:28-35). world / inverseWorld are the
+ * rigid transform and its inverse, as collectShapeColliders builds them
+ * (src/physics/fluid_sph.cpp:115-123). */
+typedef struct SphCollider {
+    uint32_t shape;          /* SphColliderShape */
+    float radius;
+    float halfExtents[3];    /* box half size; y is the half height of cylinder / capsule */
+    float pad0;
+    float world[16];
+    float inverseWorld[16];
+} SphCollider;
+
+/* physics::ForceFieldSample (src/physics/force_field.h:20-28). */
+typedef struct SphForceField {
+    float position[3];
+    float axis[3];
+    float strength;
+    float radius;
+    float falloff;
+    uint32_t type;           /* SphForceFieldType */
+} SphForceField;
+
+/* physics::FluidParams (src/physics/fluid_sph.h:42-67), field for field. */
+typedef struct SphParams {
+    float emitterWorld[16];
+    float emitterHalfExtents[3];
+    float spacing;
+    float containerMin[3];
+    float particleRadius;
+    float containerMax[3];
+    float wallRestitution;
+    float gravity[3];
+    float particleMass;
+    float smoothingRadius;
+    float restDensity;
+    float stiffness;
+    float viscosity;
+    uint32_t lattice[3];
+    uint32_t cellCount;      /* power of two */
+    uint32_t colliderCount;
+    SphCollider colliders[SPH_MAX_COLLIDERS];
+    uint32_t fieldCount;
+    SphForceField fields[SPH_MAX_FORCE_FIELDS];
+} SphParams;
+
+/* The simulation part of scene::Fluid (src/scene/scene.h:186-203) that
+ * deriveFluidParams reads. Rendering-only settings are not mirrored. */
+typedef struct SphSettings {
+    float emitterHalfExtents[3];
+    uint32_t particleCount;
+    float particleRadius;
+    float smoothingRadius;
+    float restDensity;
+    float stiffness;
+    float viscosity;
+    float containerMin[3];
+    float containerMax[3];
+    float gravity[3];
+} SphSettings;
+
+/* Neighbour-search semantics. SPH_MODE_REFERENCE reproduces the reference exactly:
+ * at most 32 particles per bucket, lowest indices first (src/physics/fluid_sph.cpp:179-187),
+ * and all 27 neighbour buckets visited even when two of them are the same bucket
+ * (src/physics/fluid_sph.cpp:205-216). The flags switch those quirks off. */
+typedef enum SphModeFlags {
+    SPH_MODE_REFERENCE = 0,
+    SPH_MODE_UNCAPPED = 1,    /* every particle of a bucket is a candidate */
+    SPH_MODE_DEDUPE = 2       /* a bucket reached from two neighbour cells is visited once */
+} SphModeFlags;
+
+/* Equation of state. LINEAR is the reference's p = max(k (rho - rho0), 0)
+ * (src/physics/fluid_sph.cpp:220). TAIT has no reference counterpart:
+ * p = max(B ((rho/rho0)^7 - 1), 0) with B = k rho0 / 7, which has the same slope
+ * dp/drho = k at rest density. */
+typedef enum SphEos {
+    SPH_EOS_LINEAR = 0,
+    SPH_EOS_TAIT = 1
+} SphEos;
+
+typedef struct SphSolver SphSolver;
+
+/* Creation options. maxParticles / maxCellCount size the device buffers once. */
+typedef struct SphSolverConfig {
+    uint32_t maxParticles;
+    uint32_t maxCellCount;   /* power of two */
+    int32_t device;          /* HIP device ordinal */
+    uint32_t flags;          /* SPH_CFG_* */
+} SphSolverConfig;
+
+#define SPH_CFG_KEEP_FORCES 1u  /* keep each particle's pressure+viscosity force of the last substep */
+#define SPH_CFG_NO_GRAPH 2u     /* launch substeps eagerly instead of replaying a captured hipGraph */
+
+/* ---- stateless helpers ---- */
+
+/* Human-readable text of the last error on the calling thread ("" if none). */
+const char* sph_last_error(void);
+
+/* physics::deriveFluidParams (src/physics/fluid_sph.cpp:65-100). The scene walk of
+ * collectShapeColliders / collectForceFields is replaced by the caller passing the
+ * already collected colliders and fields (either pointer may be NULL with count 0;
+ * counts above 8 are truncated as the reference's collectors stop at 8). */
+int sph_derive_params(const SphSettings* settings,
+                      const float emitterWorld[16],
+                      uint32_t particleCount,
+                      uint32_t cellCount,
+                      const SphCollider* colliders,
+                      uint32_t colliderCount,
+                      const SphForceField* fields,
+                      uint32_t fieldCount,
+                      SphParams* out);
+
+/* physics::fluidSubsteps (src/physics/fluid_sph.cpp:128-136). */
+uint32_t sph_substeps(const SphParams* params, float deltaSeconds);
+
+/* gfx::cellCountFor (src/gfx/fluid.cpp:50-52): clamp(bit_ceil(2n), 1024, 65536). */
+uint32_t sph_cell_count_for(uint32_t particleCount);
+
+/* physics::spatialCell + physics::spatialHash (src/physics/spatial_hash.h:12-20)
+ * evaluated on the host, for tests and tools. */
+uint32_t sph_hash_cell(int32_t cx, int32_t cy, int32_t cz, uint32_t cellCount);
+void sph_spatial_cell(const float position[3], float cellSize, int32_t outCell[3]);
+
+/* Seeded lattice jitter used by the benchmark workloads (no reference counterpart:
+ * FluidSolver::emit places an exact lattice). For each particle i in order and each
+ * component x, y, z: seed = seed * 1664525 + 1013904223 (mod 2^32),
+ * u = (seed >> 8) / 2^24, pos += (u - 0.5) * 2 * amplitude. The LCG form is the one
+ * tests/marching_cubes_test.cpp:265-269 uses. pos4 is AoS vec4, w untouched. */
+void sph_lattice_jitter(float* pos4, uint32_t count, uint32_t seed, float amplitude);
+
+/* ---- solver handle: physics::FluidSolver (src/physics/fluid_sph.h:101-127) ---- */
+
+int sph_create(const SphSolverConfig* config, SphSolver** out);
+int sph_destroy(SphSolver* solver);
+
+/* FluidSolver::emit (src/physics/fluid_sph.cpp:138-154): lattice positions, w = rest
+ * density, zero velocity, previous = current. Runs on the device. */
+int sph_emit(SphSolver* solver, const SphParams* params, uint32_t count);
+
+/* FluidSolver::step (src/physics/fluid_sph.cpp:311-321): fluidSubsteps(params, dt)
+ * substeps of min(dt, 1/30) / n each; no-op on an empty solver or dt <= 0.
+ * Enqueued on the solver's stream; returns before the device finishes. */
+int sph_step(SphSolver* solver, const SphParams* params, float deltaSeconds);
+
+/* One FluidSolver::substep (src/physics/fluid_sph.cpp:190-309) of length dt. */
+int sph_substep(SphSolver* solver, const SphParams* params, float dt);
+
+/* Block until the solver's stream is idle. */
+int sph_synchronize(SphSolver* solver);
+
+/* FluidSolver::particleCount (src/physics/fluid_sph.h:108). */
+uint32_t sph_particle_count(const SphSolver* solver);
+
+/* FluidSolver::particles (src/physics/fluid_sph.h:110) plus the velocities the reference
+ * keeps privately (src/physics/fluid_sph.h:121-122). Either pointer may be NULL.
+ * Synchronous. Buffers hold count * 4 floats. */
+int sph_read_particles(SphSolver* solver, float* pos4, float* vel4);
+
+/* FluidSolver::previousParticles / keepRendered (src/physics/fluid_sph.h:112-114). */
+int sph_read_previous(SphSolver* solver, float* pos4);
+int sph_keep_rendered(SphSolver* solver);
+
+/* ---- extensions the reference has no counterpart for ---- */
+
+/* Replace the state with caller data (index order, AoS vec4). previous = positions. */
+int sph_load_state(SphSolver* solver, const float* pos4, const float* vel4, uint32_t count);
+
+/* The FluidGrid of the last substep in the reference's format
+ * (src/physics/fluid_sph.h:84-97): cellOf[count] by particle index, cellCounts[cellCount]
+ * uncapped, cellParticles[cellCount * 32] with the first min(count, 32) slots of each
+ * bucket valid (the rest are 0xFFFFFFFF). Any pointer may be NULL. Synchronous. */
+int sph_read_grid(SphSolver* solver, uint32_t* cellOf, uint32_t* cellCounts, uint32_t* cellParticles);
+
+/* The bucket-sorted layout behind the grid: keys[count] and ids[count] in sorted order,
+ * starts[cellCount + 1]. Any pointer may be NULL. Synchronous. */
+int sph_read_sorted(SphSolver* solver, uint32_t* keys, uint32_t* ids, uint32_t* starts);
+
+/* Pressure + viscosity force of the last substep, per particle in index order, as
+ * (fx, fy, fz, 0) — the `force` accumulator of src/physics/fluid_sph.cpp:233-258, which
+ * the reference never stores. Requires SPH_CFG_KEEP_FORCES. Synchronous. */
+int sph_read_forces(SphSolver* solver, float* force4);
+
+/* SphModeFlags / SphEos. Default: SPH_MODE_REFERENCE, SPH_EOS_LINEAR. */
+int sph_set_mode(SphSolver* solver, uint32_t modeFlags);
+int sph_set_eos(SphSolver* solver, uint32_t eos);
+
+/* Per-step conservation sums in FP64 over the current state: total mass
+ * count * particleMass and E = sum(0.5 m |v|^2 - m g.x) (kinetic plus gravitational
+ * potential). out[0] = mass, out[1] = kinetic, out[2] = potential. Synchronous. */
+int sph_conservation(SphSolver* solver, const SphParams* params, double out[3]);
+
+/* Device time, in milliseconds, of each kernel class over the substeps run since the
+ * last reset, measured with HIP events on the solver's stream when profiling is on.
+ * names/ms/launches hold up to `capacity` entries; returns the number written. */
+int sph_profile_enable(SphSolver* solver, int enable);
+int sph_profile_read(SphSolver* solver, const char** names, double* ms, uint32_t* launches, uint32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_C_API_H */

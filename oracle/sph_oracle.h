@@ -1,0 +1,89 @@
+/*
+ * sph_oracle.h — CPU oracle for the SPH fluid step. TEST INFRASTRUCTURE ONLY.
+ *
+ * Only the test suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load this library, and only as the checker / the timed CPU baseline. The product
+ * path (computer_graphics_lab_00_amd) never calls it.
+ *
+ * Two restatements of physics::FluidSolver (src/physics/fluid_sph.cpp):
+ *   orc32_*  — FP32, operation for operation in the reference's order (glm's scalar
+ *              code path: dot = (x*x + y*y) + z*z, vec/scalar = per-component IEEE
+ *              division, mat4*vec4 = (m0 x + m1 y) + (m2 z + m3 w)), compiled with
+ *              -ffp-contract=off and no fast-math. This is the parity judge.
+ *   orc64_*  — the same algorithm in FP64, for error statements. Its grid keys are
+ *              computed with the FP32 rule on FP32-rounded positions so that both
+ *              oracles see the same neighbour sets; everything else is FP64.
+ *
+ * Parity status per function (what pins each one) is listed in the header of
+ * sph_oracle.c.
+ */
+#ifndef SPH_ORACLE_H
+#define SPH_ORACLE_H
+
+#include <stdint.h>
+
+#include "sph_c_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* stateless pieces */
+uint32_t orc_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t cellCount);
+void orc_cell(const float p[3], float h, int32_t out[3]);
+uint32_t orc_substeps(const SphParams* params, float dt);
+uint32_t orc_cell_count_for(uint32_t n);
+void orc_derive_params(const SphSettings* s, const float world[16], uint32_t n, uint32_t cellCount,
+                       const SphCollider* colliders, uint32_t nc, const SphForceField* fields, uint32_t nf,
+                       SphParams* out);
+float orc_poly6_f32(float r2, float h);
+double orc_poly6_f64(double r2, double h);
+void orc_spiky_gradient_f32(const float delta[3], float len, float h, float out[3]);
+float orc_visc_laplacian_f32(float len, float h);
+void orc_field_accel_f32(const SphForceField* fields, uint32_t count, const float p[3], float out[3]);
+/* closestOnColliderLocal (src/physics/collider_shapes.h:115-131): out = point[3], normal[3], distance */
+void orc_closest_on_collider_f32(uint32_t shape, float radius, const float half[3], const float p[3], float out[7]);
+void orc_lattice_jitter(float* pos4, uint32_t count, uint32_t seed, float amplitude);
+
+/* solver state */
+typedef struct OrcSolver32 OrcSolver32;
+typedef struct OrcSolver64 OrcSolver64;
+
+OrcSolver32* orc32_create(void);
+void orc32_destroy(OrcSolver32* s);
+void orc32_emit(OrcSolver32* s, const SphParams* p, uint32_t count);
+void orc32_load_state(OrcSolver32* s, const float* pos4, const float* vel4, uint32_t count);
+void orc32_set_mode(OrcSolver32* s, uint32_t modeFlags, uint32_t eos);
+/* threads: 0 = serial (JobSystem* == nullptr), otherwise that many threads including the caller */
+void orc32_step(OrcSolver32* s, const SphParams* p, float dt, uint32_t threads);
+void orc32_substep(OrcSolver32* s, const SphParams* p, float dt, uint32_t threads);
+uint32_t orc32_count(const OrcSolver32* s);
+void orc32_read(const OrcSolver32* s, float* pos4, float* vel4);
+void orc32_read_previous(const OrcSolver32* s, float* pos4);
+void orc32_keep_rendered(OrcSolver32* s);
+void orc32_read_forces(const OrcSolver32* s, float* force4);
+void orc32_read_grid(const OrcSolver32* s, uint32_t* cellOf, uint32_t* cellCounts, uint32_t* cellParticles);
+
+OrcSolver64* orc64_create(void);
+void orc64_destroy(OrcSolver64* s);
+void orc64_emit(OrcSolver64* s, const SphParams* p, uint32_t count);
+void orc64_load_state(OrcSolver64* s, const float* pos4, const float* vel4, uint32_t count);
+void orc64_set_mode(OrcSolver64* s, uint32_t modeFlags, uint32_t eos);
+void orc64_step(OrcSolver64* s, const SphParams* p, float dt, uint32_t threads);
+void orc64_substep(OrcSolver64* s, const SphParams* p, float dt, uint32_t threads);
+uint32_t orc64_count(const OrcSolver64* s);
+void orc64_read(const OrcSolver64* s, double* pos4, double* vel4);
+void orc64_read_forces(const OrcSolver64* s, double* force4);
+void orc64_read_grid(const OrcSolver64* s, uint32_t* cellOf, uint32_t* cellCounts, uint32_t* cellParticles);
+
+/* FP64 conservation sums over a float state: out = {mass, kinetic, potential}. */
+void orc_conservation(const float* pos4, const float* vel4, uint32_t count, const SphParams* p, double out[3]);
+
+/* number of hardware threads the baseline pool would use (hardware_concurrency) */
+uint32_t orc_hardware_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
