@@ -30,10 +30,12 @@ HBM_PEAK_GBS = 8000.0
 DT = 1.0 / 600.0
 
 # Algorithmic bytes per particle-substep (FP32, counting sort with one reorder, neighbour
-# reads served on chip and not counted): keys 16, rank/scatter 8, reorder 52, density 20,
-# force/integrate 56 (read pos, vel, density, pressure 32 + write pos, vel 24), plus the
-# per-bucket counts/starts 8 C / N.
-BYTES_FORCE = 56.0
+# reads served on chip and not counted): keys 16 (read pos 12 + write key 4), rank/scatter
+# 8, reorder 52 (read index 4 + pos, vel 24 + write pos, vel 24), density 20 (read pos 12
+# + write density, pressure 8), force/integrate 56 (read pos, vel, density, pressure 32 +
+# write pos, vel 24), plus the per-bucket counts/starts 8 C / N.
+BYTES_PER_PARTICLE = {"keys": 16.0, "scatter": 8.0, "rank_reorder": 52.0, "density": 20.0,
+                      "force_integrate": 56.0}
 BYTES_SUBSTEP_FIXED = 152.0
 
 
@@ -45,6 +47,10 @@ def parse():
     ap.add_argument("--cells", type=int, default=1 << 21)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bucket-walk", action="store_true",
+                    help="force pass walks the 27 buckets instead of the density pass's neighbour list")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one 256 MiB device copy kernel first (byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -77,11 +83,13 @@ def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
     }
 
 
-def load_traffic():
-    f = ROOT / "profiles" / "traffic_force_integrate_dam_break_1m.json"
+def load_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (counters scaled by a plain copy kernel of known size), or None."""
+    f = ROOT / "profiles" / "traffic_dam_break_1m.json"
     if f.exists():
         try:
-            return json.loads(f.read_text()).get("bytes_per_launch")
+            return json.loads(f.read_text())["kernels"][kernel]["bytes_per_launch"]
         except Exception:
             return None
     return None
@@ -94,7 +102,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from computer_graphics_lab_00_amd import fluid
+    from computer_graphics_lab_00_amd import capi, fluid
+    if args.calibrate:
+        capi.check(capi.load().sph_debug_copy(256 << 20))
 
     w = fluid.m1(args.cells)
     p = w.params()
@@ -104,7 +114,7 @@ def main():
     print(f"[bench] workload {w.name}: {n} particles, {p.cellCount} buckets, {substeps} substeps per step",
           file=sys.stderr, flush=True)
 
-    gpu = fluid.FluidSolver(n, max(p.cellCount, 1024))
+    gpu = fluid.FluidSolver(n, max(p.cellCount, 1024), flags=capi.CFG_BUCKET_WALK if args.bucket_walk else 0)
     gpu.load_state(pos, vel)
     for _ in range(args.warmup):
         gpu.step(p, DT)
@@ -131,7 +141,7 @@ def main():
     dom = max(kernel_ms, key=kernel_ms.get)
     launches = prof[dom][1]
     avg_ms = prof[dom][0] / max(launches, 1)
-    per_unit = BYTES_FORCE if dom == "force_integrate" else None
+    per_unit = BYTES_PER_PARTICLE.get(dom)
     achieved = (per_unit * n / (avg_ms * 1e-3)) / 1e9 if per_unit else None
     substep_ms = sum(kernel_ms.values()) / max(prof["force_integrate"][1], 1)
     b_alg = BYTES_SUBSTEP_FIXED + 8.0 * p.cellCount / n
@@ -142,7 +152,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-        "traffic": load_traffic(),
+        "traffic": load_traffic(dom),
         "bytes_per_unit": per_unit,
         "units_per_launch": n,
         "avg_launch_ms": avg_ms,
@@ -173,6 +183,7 @@ def main():
         "config": {"workload": w.name, "particles": n, "lattice": list(w.lattice), "spacing": w.spacing,
                    "smoothing_radius": w.smoothing, "cell_count": p.cellCount, "substeps_per_step": substeps,
                    "dt": DT, "eos": "linear k=2000", "mode": "reference (32-slot cap, duplicate visits)",
+                   "neighbour_list": not args.bucket_walk,
                    "particle_substeps_per_s": value * substeps},
         "roofline": roofline,
         "cpu_baseline": cpu,

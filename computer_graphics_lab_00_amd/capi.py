@@ -35,6 +35,7 @@ MODE_REFERENCE, MODE_UNCAPPED, MODE_DEDUPE = 0, 1, 2
 EOS_LINEAR, EOS_TAIT = 0, 1
 CFG_KEEP_FORCES = 1
 CFG_NO_GRAPH = 2
+CFG_BUCKET_WALK = 4
 
 
 class SphCollider(C.Structure):
@@ -174,6 +175,7 @@ def _declare(lib):
         "sph_profile_enable": (C.c_int, [P, C.c_int]),
         "sph_profile_read": (C.c_int, [P, C.POINTER(C.c_char_p), _DP, _UP, C.c_uint32]),
         "sph_debug_step_constants": (C.c_int, [C.POINTER(SphParams), C.c_float, _FP]),
+        "sph_debug_copy": (C.c_int, [C.c_uint64]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -26,6 +26,7 @@ constexpr float kPi = 3.14159265358979323846f;   // std::numbers::pi_v<float>
 enum KernelClass : uint32_t {
     KC_FILL = 0,
     KC_KEYS,
+    KC_BINSLOT,
     KC_SCAN,
     KC_SCATTER,
     KC_RANK,
@@ -34,8 +35,8 @@ enum KernelClass : uint32_t {
     KC_COUNT
 };
 
-const char* const kKernelNames[KC_COUNT] = {"fill_counts", "keys", "scan", "scatter", "rank_reorder", "density",
-                                            "force_integrate"};
+const char* const kKernelNames[KC_COUNT] = {"grid_clear", "keys", "bin_slots", "scan", "scatter", "rank_reorder",
+                                            "density", "force_integrate"};
 
 struct TimedLaunch {
     uint32_t cls;
@@ -57,12 +58,20 @@ struct SphSolver {
     uint32_t* ids[2] = {nullptr, nullptr};
     int pc = 0;   // pos/vel buffer holding the state
     int ic = 0;   // ids buffer matching the state's order
+    uint32_t maxBins = 0;
     uint32_t* keys = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t* rep = nullptr;
+    uint32_t* bucketStart = nullptr;
+    uint32_t* binStart = nullptr;
+    uint32_t* binOf = nullptr;
     uint32_t* slots = nullptr;
-    uint32_t* starts = nullptr;
     uint32_t* blockSums = nullptr;
     uint32_t* keysSorted = nullptr;
-    uint2* pairs = nullptr;
+    uint2* pairKI = nullptr;
+    uint32_t* pairSrc = nullptr;
+    uint4* nbrList = nullptr;       // kNeighbourCap / 4 groups x maxN
+    uint32_t* nbrCount = nullptr;
     float4* prev = nullptr;
     float4* force = nullptr;
     float4* staging = nullptr;
@@ -177,7 +186,48 @@ StepConsts make_consts(const SphSolver* s, const SphParams& p, float dt) {
     k.restitution = p.wallRestitution;
     k.colliderCount = p.colliderCount < SPH_MAX_COLLIDERS ? p.colliderCount : SPH_MAX_COLLIDERS;
     k.fieldCount = p.fieldCount < SPH_MAX_FORCE_FIELDS ? p.fieldCount : SPH_MAX_FORCE_FIELDS;
+    // The force pass may reuse the density pass's in-range list only if its own cut-off
+    // (r2 < r2Cut) selects a subset of the density's (r2 < h2).
+    k.listCap = (s->nbrList && k.r2Cut <= k.h2) ? sphdev::kNeighbourCap : 0u;
     return k;
+}
+
+// Spatial bins over the container (plus one cell of margin each side). Any deterministic
+// mapping is correct here; this one keeps neighbouring cells close in memory. If the
+// container holds more cells than the bin arrays, bins are coarsened by powers of two.
+sphdev::BinGrid make_bins(const SphSolver* s, const SphParams& p) {
+    sphdev::BinGrid g;
+    memset(&g, 0, sizeof(g));
+    const double h = p.smoothingRadius;
+    int64_t lo[3], span[3];
+    for (int a = 0; a < 3; ++a) {
+        double mn = p.containerMin[a] < p.containerMax[a] ? p.containerMin[a] : p.containerMax[a];
+        double mx = p.containerMin[a] < p.containerMax[a] ? p.containerMax[a] : p.containerMin[a];
+        double l = floor(mn / h) - 1.0, u = floor(mx / h) + 1.0;
+        if (!(l > -1e9)) l = -1e9;
+        if (!(u < 1e9)) u = 1e9;
+        if (u < l) u = l;
+        lo[a] = (int64_t)l;
+        span[a] = (int64_t)u - (int64_t)l + 1;
+    }
+    uint32_t shift = 0;
+    for (;;) {
+        uint64_t tiles = 1;
+        for (int a = 0; a < 3; ++a) {
+            const uint64_t d = (uint64_t)((span[a] + (1ll << shift) - 1) >> shift);
+            g.dims[a] = (uint32_t)d;
+            g.tiles[a] = (uint32_t)((d + 3) / 4);
+            tiles *= g.tiles[a];
+        }
+        if (tiles * 64ull <= s->maxBins || shift >= 31) {
+            g.bins = (uint32_t)(tiles * 64ull);
+            break;
+        }
+        ++shift;
+    }
+    g.shift = shift;
+    for (int a = 0; a < 3; ++a) g.origin[a] = (int32_t)lo[a];
+    return g;
 }
 
 void free_all(SphSolver* s) {
@@ -187,11 +237,18 @@ void free_all(SphSolver* s) {
         hipFree(s->ids[i]);
     }
     hipFree(s->keys);
+    hipFree(s->counts);
+    hipFree(s->rep);
+    hipFree(s->bucketStart);
+    hipFree(s->binStart);
+    hipFree(s->binOf);
     hipFree(s->slots);
-    hipFree(s->starts);
     hipFree(s->blockSums);
     hipFree(s->keysSorted);
-    hipFree(s->pairs);
+    hipFree(s->pairKI);
+    hipFree(s->pairSrc);
+    hipFree(s->nbrList);
+    hipFree(s->nbrCount);
     hipFree(s->prev);
     hipFree(s->force);
     hipFree(s->staging);
@@ -214,31 +271,41 @@ int check_params(const SphSolver* s, const SphParams* p) {
 
 int run_substep(SphSolver* s, const SphParams& p, float dt) {
     const StepConsts k = make_consts(s, p, dt);
+    const sphdev::BinGrid bins = make_bins(s, p);
     const uint32_t C = p.cellCount;
     sphdev::GridBuffers g;
     g.pos = s->pos[s->pc];
     g.vel = s->vel[s->pc];
     g.ids = s->ids[s->ic];
     g.keys = s->keys;
+    g.counts = s->counts;
+    g.rep = s->rep;
+    g.binStart = s->binStart;
+    g.binOf = s->binOf;
     g.slots = s->slots;
-    g.starts = s->starts;
-    g.pairs = s->pairs;
+    g.pairKI = s->pairKI;
+    g.pairSrc = s->pairSrc;
     g.blockSums = s->blockSums;
     g.posSorted = s->pos[1 - s->pc];
     g.velSorted = s->vel[1 - s->pc];
     g.idsSorted = s->ids[1 - s->ic];
     g.keysSorted = s->keysSorted;
+    g.bucketStart = s->bucketStart;
     {
         Timer t(s, KC_FILL);
-        SPH_HIP(sphdev::launch_fill_u32(s->starts, 0u, C, s->stream));
+        SPH_HIP(sphdev::launch_grid_clear(k, bins, g, s->stream));
     }
     {
         Timer t(s, KC_KEYS);
-        SPH_HIP(sphdev::launch_keys(k, g, s->stream));
+        SPH_HIP(sphdev::launch_keys(k, bins, g, s->stream));
+    }
+    {
+        Timer t(s, KC_BINSLOT);
+        SPH_HIP(sphdev::launch_bin_slots(k, g, s->stream));
     }
     {
         Timer t(s, KC_SCAN);
-        SPH_HIP(sphdev::launch_scan(s->starts, C, s->n, s->blockSums, s->stream));
+        SPH_HIP(sphdev::launch_scan(s->binStart, bins.bins, s->n, s->blockSums, s->stream));
     }
     {
         Timer t(s, KC_SCATTER);
@@ -250,7 +317,9 @@ int run_substep(SphSolver* s, const SphParams& p, float dt) {
     }
     {
         Timer t(s, KC_DENSITY);
-        SPH_HIP(sphdev::launch_density(k, g.posSorted, g.velSorted, s->starts, s->stream));
+        SPH_HIP(sphdev::launch_density(k, g.posSorted, g.velSorted, s->bucketStart, s->counts,
+                                       k.listCap ? s->nbrList : nullptr,
+                                       k.listCap ? s->nbrCount : nullptr, s->stream));
     }
     {
         sphdev::BoundaryConsts b;
@@ -258,7 +327,9 @@ int run_substep(SphSolver* s, const SphParams& p, float dt) {
         memcpy(b.colliders, p.colliders, sizeof(b.colliders));
         memcpy(b.fields, p.fields, sizeof(b.fields));
         Timer t(s, KC_FORCE);
-        SPH_HIP(sphdev::launch_force(k, b, g.posSorted, g.velSorted, s->starts, s->pos[s->pc], s->vel[s->pc],
+        SPH_HIP(sphdev::launch_force(k, b, g.posSorted, g.velSorted, s->bucketStart, s->counts,
+                                     k.listCap ? s->nbrList : nullptr,
+                                     k.listCap ? s->nbrCount : nullptr, s->pos[s->pc], s->vel[s->pc],
                                      (s->flags & SPH_CFG_KEEP_FORCES) ? s->force : nullptr, s->stream));
     }
     s->ic = 1 - s->ic;
@@ -291,7 +362,11 @@ extern "C" int sph_create(const SphSolverConfig* cfg, SphSolver** out) {
     s->maxC = cfg->maxCellCount;
     s->flags = cfg->flags;
     const size_t N = cfg->maxParticles, C = cfg->maxCellCount;
-    const size_t tiles = (C + sphdev::kScanTile - 1) / sphdev::kScanTile;
+    // Spatial bins: about 4 per particle of capacity, between 2^16 and 2^23.
+    size_t nb = 1u << 16;
+    while (nb < 4 * N && nb < (1u << 23)) nb <<= 1;
+    s->maxBins = (uint32_t)nb;
+    const size_t tiles = (nb + sphdev::kScanTile - 1) / sphdev::kScanTile;
     hipError_t e = hipSetDevice(s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
@@ -300,11 +375,20 @@ extern "C" int sph_create(const SphSolverConfig* cfg, SphSolver** out) {
         if (e == hipSuccess) e = hipMalloc(&s->ids[i], N * sizeof(uint32_t));
     }
     if (e == hipSuccess) e = hipMalloc(&s->keys, N * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s->counts, C * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s->rep, C * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s->bucketStart, C * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s->binStart, (nb + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s->binOf, N * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&s->slots, N * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&s->starts, (C + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&s->blockSums, (tiles + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&s->keysSorted, N * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&s->pairs, N * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc(&s->pairKI, N * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc(&s->pairSrc, N * sizeof(uint32_t));
+    if (e == hipSuccess && !(cfg->flags & SPH_CFG_BUCKET_WALK)) {
+        e = hipMalloc(&s->nbrList, N * (sphdev::kNeighbourCap / 4) * sizeof(uint4));
+        if (e == hipSuccess) e = hipMalloc(&s->nbrCount, N * sizeof(uint32_t));
+    }
     if (e == hipSuccess) e = hipMalloc(&s->prev, N * sizeof(float4));
     if (e == hipSuccess && (cfg->flags & SPH_CFG_KEEP_FORCES)) e = hipMalloc(&s->force, N * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&s->staging, N * sizeof(float4));
@@ -449,13 +533,14 @@ extern "C" int sph_read_grid(SphSolver* s, uint32_t* cellOf, uint32_t* cellCount
         SPH_HIP(hipStreamSynchronize(s->stream));
     }
     if (cellCounts || cellParticles) {
-        std::vector<uint32_t> starts((size_t)C + 1);
+        std::vector<uint32_t> counts(C), starts(C);
         std::vector<uint32_t> ids(n);
-        SPH_HIP(hipMemcpyAsync(starts.data(), s->starts, ((size_t)C + 1) * 4, hipMemcpyDeviceToHost, s->stream));
+        SPH_HIP(hipMemcpyAsync(counts.data(), s->counts, (size_t)C * 4, hipMemcpyDeviceToHost, s->stream));
+        SPH_HIP(hipMemcpyAsync(starts.data(), s->bucketStart, (size_t)C * 4, hipMemcpyDeviceToHost, s->stream));
         SPH_HIP(hipMemcpyAsync(ids.data(), s->ids[s->ic], (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
         SPH_HIP(hipStreamSynchronize(s->stream));
         for (uint32_t b = 0; b < C; ++b) {
-            const uint32_t cnt = starts[b + 1] - starts[b];
+            const uint32_t cnt = counts[b];
             if (cellCounts) cellCounts[b] = cnt;
             if (cellParticles) {
                 for (uint32_t k = 0; k < SPH_CELL_CAPACITY; ++k)
@@ -466,15 +551,22 @@ extern "C" int sph_read_grid(SphSolver* s, uint32_t* cellOf, uint32_t* cellCount
     return SPH_OK;
 }
 
-extern "C" int sph_read_sorted(SphSolver* s, uint32_t* keys, uint32_t* ids, uint32_t* starts) {
+extern "C" int sph_read_sorted(SphSolver* s, uint32_t* keys, uint32_t* ids, uint32_t* bucketStarts) {
     sphhost::clear_error();
     if (!s) return sphhost::fail(SPH_ERR_INVALID_ARGUMENT, "sph_read_sorted: null solver");
     if (!s->hasGrid) return sphhost::fail(SPH_ERR_STATE, "no grid: run a substep first");
     if (int st = set_device(s)) return st;
     if (keys) SPH_HIP(hipMemcpyAsync(keys, s->keysSorted, (size_t)s->n * 4, hipMemcpyDeviceToHost, s->stream));
     if (ids) SPH_HIP(hipMemcpyAsync(ids, s->ids[s->ic], (size_t)s->n * 4, hipMemcpyDeviceToHost, s->stream));
-    if (starts)
-        SPH_HIP(hipMemcpyAsync(starts, s->starts, ((size_t)s->gridC + 1) * 4, hipMemcpyDeviceToHost, s->stream));
+    if (bucketStarts) {
+        const uint32_t C = s->gridC;
+        std::vector<uint32_t> counts(C);
+        SPH_HIP(hipMemcpyAsync(counts.data(), s->counts, (size_t)C * 4, hipMemcpyDeviceToHost, s->stream));
+        SPH_HIP(hipMemcpyAsync(bucketStarts, s->bucketStart, (size_t)C * 4, hipMemcpyDeviceToHost, s->stream));
+        SPH_HIP(hipStreamSynchronize(s->stream));
+        for (uint32_t b = 0; b < C; ++b)
+            if (counts[b] == 0) bucketStarts[b] = 0xFFFFFFFFu;
+    }
     SPH_HIP(hipStreamSynchronize(s->stream));
     return SPH_OK;
 }
@@ -573,5 +665,22 @@ extern "C" int sph_debug_step_constants(const SphParams* p, float dt, float out[
     out[5] = k.maxSpeed;
     out[6] = k.negMass;
     out[7] = k.viscMass;
+    return SPH_OK;
+}
+
+// Profiling aid: one plain device-to-device copy of `bytes` bytes by a kernel, so that
+// hardware byte counters can be scaled against a transfer of known size.
+extern "C" int sph_debug_copy(uint64_t bytes) {
+    float4* a = nullptr;
+    float4* b = nullptr;
+    const size_t n = (size_t)(bytes / sizeof(float4));
+    SPH_HIP(hipMalloc(&a, n * sizeof(float4)));
+    SPH_HIP(hipMalloc(&b, n * sizeof(float4)));
+    hipMemset(a, 0, n * sizeof(float4));
+    hipError_t e = sphdev::launch_calibration_copy(a, b, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return hip_fail(e, "sph_debug_copy");
     return SPH_OK;
 }

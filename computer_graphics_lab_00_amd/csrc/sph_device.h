@@ -1,13 +1,15 @@
 // sph_device.h — device-side types shared by the SPH kernels and the host driver.
 //
 // State layout in HBM (one solver):
-//   pos4[n]  float4 (x, y, z, density)     bucket-sorted order, ping-pong pair
-//   vel4[n]  float4 (vx, vy, vz, pressure) bucket-sorted order, ping-pong pair
-//   ids[n]   uint32 particle index of each sorted slot, ping-pong pair
-//   starts[C + 1]  first sorted slot of each hash bucket (exclusive scan of counts)
-// Inside a bucket the slots are in ascending particle index, which is the order the
-// reference's serial insertion produces (src/physics/fluid_sph.cpp:181-187), so the
-// reference's `cellParticles[b * 32 + k]` is `ids[starts[b] + k]` for k < min(count, 32).
+//   pos4[n]  float4 (x, y, z, density)     ping-pong pair
+//   vel4[n]  float4 (vx, vy, vz, pressure) ping-pong pair
+//   ids[n]   uint32 particle index of each slot, ping-pong pair
+//   counts[C]       members of each hash bucket (the reference's uncapped cellCounts)
+//   bucketStart[C]  first slot of each non-empty bucket
+// Slots are ordered by space (spatial bin of the bucket), then bucket, then particle
+// index. Every bucket is contiguous and ascending in index — the order of the reference's
+// serial insertion (src/physics/fluid_sph.cpp:181-187) — so its
+// `cellParticles[b * 32 + k]` is `ids[bucketStart[b] + k]` for k < min(counts[b], 32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +22,7 @@ namespace sphdev {
 constexpr uint32_t kBlock = 256;          // 4 wave64 per workgroup
 constexpr uint32_t kScanPerThread = 8;    // elements each scan thread owns
 constexpr uint32_t kScanTile = kBlock * kScanPerThread;
+constexpr uint32_t kXcds = 8;             // MI355X: 8 XCDs, each with its own L2
 
 // Scalars of one substep, precomputed on the host exactly as the reference computes
 // them (src/physics/fluid_sph.cpp:24-46, :194-195, :264, :271-272) so device and
@@ -50,7 +53,23 @@ struct StepConsts {
     float restitution;
     uint32_t colliderCount;
     uint32_t fieldCount;
+    uint32_t listCap;       // neighbour-list slots per particle (0: no list)
 };
+
+// Spatial bins that order the buckets in memory: the container's cells (plus a margin),
+// coarsened by 2^shift per axis if needed, in 4x4x4 tiles.
+struct BinGrid {
+    int32_t origin[3];      // cell coordinate of bin (0, 0, 0)
+    uint32_t dims[3];       // bins per axis
+    uint32_t tiles[3];      // 4-bin tiles per axis
+    uint32_t shift;
+    uint32_t bins;          // tiles product * 64
+};
+
+// Neighbour-list slots per particle: the in-range count of the reference's kernel
+// (h = 2 spacing) is about 33, 45 in compressed fluid; beyond the cap the force pass
+// falls back to walking the buckets.
+constexpr uint32_t kNeighbourCap = 64;
 
 // Colliders and force fields travel in kernel arguments (at most 8 each).
 struct BoundaryConsts {
@@ -59,36 +78,55 @@ struct BoundaryConsts {
 };
 
 struct GridBuffers {
-    const float4* pos;      // state before the sort (any order)
+    const float4* pos;      // state before the sort (previous layout)
     const float4* vel;
     const uint32_t* ids;
-    uint32_t* keys;         // per pre-sort slot
-    uint32_t* slots;        // atomic slot inside the bucket (arbitrary order)
-    uint32_t* starts;       // C + 1
-    uint2* pairs;           // (particle id, pre-sort slot) scattered by bucket
+    uint32_t* keys;         // hash key per pre-sort slot
+    uint32_t* counts;       // C
+    uint32_t* rep;          // C: smallest spatial bin among each bucket's cells
+    uint32_t* binStart;     // bins + 1: counts, then their exclusive scan
+    uint32_t* binOf;        // per pre-sort slot
+    uint32_t* slots;        // provisional slot inside the bin (arbitrary order)
+    uint2* pairKI;          // (key, id) scattered by bin
+    uint32_t* pairSrc;      // pre-sort slot of each scattered pair
     uint32_t* blockSums;
     float4* posSorted;
     float4* velSorted;
     uint32_t* idsSorted;
     uint32_t* keysSorted;
+    uint32_t* bucketStart;  // C
 };
+
+// Logical block for a hardware workgroup id. Workgroups are dispatched round-robin over
+// the XCDs; this gives XCD x a contiguous range of logical blocks (slots), so each L2
+// caches one spatially compact slab of the state.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t blocks) {
+    const uint32_t q = blocks / kXcds, r = blocks % kXcds;
+    const uint32_t x = hw % kXcds, k = hw / kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
 
 // launchers (sph_grid.hip)
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t s);
-hipError_t launch_keys(const StepConsts& k, const GridBuffers& g, hipStream_t s);
+hipError_t launch_grid_clear(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s);
+hipError_t launch_keys(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s);
+hipError_t launch_bin_slots(const StepConsts& k, const GridBuffers& b, hipStream_t s);
 hipError_t launch_scan(uint32_t* data, uint32_t count, uint32_t total, uint32_t* blockSums, hipStream_t s);
-hipError_t launch_scatter(const StepConsts& k, const GridBuffers& g, hipStream_t s);
-hipError_t launch_rank_reorder(const StepConsts& k, const GridBuffers& g, hipStream_t s);
+hipError_t launch_scatter(const StepConsts& k, const GridBuffers& b, hipStream_t s);
+hipError_t launch_rank_reorder(const StepConsts& k, const GridBuffers& b, hipStream_t s);
 hipError_t launch_unpermute(const float4* src, const uint32_t* ids, float4* dst, uint32_t n, hipStream_t s);
 hipError_t launch_gather_keys(const uint32_t* keysSorted, const uint32_t* ids, uint32_t* cellOf, uint32_t n,
                               hipStream_t s);
 hipError_t launch_emit(const SphParams& p, uint32_t count, float4* pos, float4* vel, float4* prev, uint32_t* ids,
                        hipStream_t s);
 hipError_t launch_iota(uint32_t* ids, uint32_t n, hipStream_t s);
+hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 
 // launchers (sph_forces.hip)
-hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* starts, hipStream_t s);
+hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* bucketStart,
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, hipStream_t s);
 hipError_t launch_force(const StepConsts& k, const BoundaryConsts& b, const float4* pos, const float4* vel,
-                        const uint32_t* starts, float4* posOut, float4* velOut, float4* forceOut, hipStream_t s);
+                        const uint32_t* bucketStart, const uint32_t* counts, const uint4* nbrList,
+                        const uint32_t* nbrCount, float4* posOut, float4* velOut, float4* forceOut, hipStream_t s);
 
 }  // namespace sphdev

@@ -23,6 +23,20 @@ __device__ inline uint32_t hash_cell(int32_t cx, int32_t cy, int32_t cz, uint32_
 
 __device__ inline int32_t cell_coord(float x, float h) { return (int32_t)floorf(x / h); }
 
+// Streaming (non-temporal) 16-byte accesses for the neighbour list, which is written once
+// and read once per substep and should not push particle state out of L2.
+typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void store_stream(uint4* p, uint4 v) {
+    u32x4_nt x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<u32x4_nt*>(p));
+}
+
+__device__ __forceinline__ uint4 load_stream(const uint4* p) {
+    const u32x4_nt x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_nt*>(p));
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
+
 // Whether the bucket of neighbour offset `o` (0..26) already appeared at an earlier offset.
 __device__ inline bool seen_before(int32_t cx, int32_t cy, int32_t cz, uint32_t o, uint32_t b, uint32_t mask) {
     for (uint32_t e = 0; e < o; ++e) {
@@ -46,9 +60,19 @@ __device__ inline float eos_pressure(const StepConsts& k, float density) {
     return p < 0.0f ? 0.0f : p;
 }
 
+// Density pass. Besides density and pressure it records, per particle, the slots of
+// every candidate with r2 < h2 in visiting order (duplicates included). The force
+// pass's candidates are a subset of these (its cut-off is r2 < r2Cut <= h2 on the same
+// positions), so it walks ~33 entries instead of the 27 buckets. Entries are kept four
+// at a time in registers and stored as one 16-byte group; group g of particle t sits
+// at groups[g * n + t], so the force pass reads a whole wave's group g in one coalesced
+// access. A particle with more than `listCap` entries stores count = listCap + 1 and the
+// force pass scans its buckets instead.
 __global__ void __launch_bounds__(kBlock)
-    k_density(StepConsts k, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ starts) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    k_density(StepConsts k, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
+              const uint32_t* __restrict__ counts,
+              uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount) {
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= k.n) return;
     // Only xyz of any slot is read here; .w of this lane's own slot is written below
     // (the reference writes positions[i].w in place the same way, fluid_sph.cpp:197-199).
@@ -57,24 +81,54 @@ __global__ void __launch_bounds__(kBlock)
     const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
     const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
     const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
+    const uint32_t cap = nbrList ? k.listCap : 0u;
     float density = 0.0f;
-    uint32_t o = 0;
+    uint32_t listed = 0;
+    uint4 group = make_uint4(0u, 0u, 0u, 0u);
     for (int32_t dz = -1; dz <= 1; ++dz) {
-        for (int32_t dy = -1; dy <= 1; ++dy) {
-            for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
-                const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
-                if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
-                const uint32_t lo = starts[b];
-                uint32_t cnt = starts[b + 1] - lo;
-                if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
-                const float* q = reinterpret_cast<const float*>(pos + lo);
-                for (uint32_t j = 0; j < cnt; ++j, q += 4) {
-                    const float ddx = px - q[0], ddy = py - q[1], ddz = pz - q[2];
+        // The nine buckets of one z layer: all descriptors are fetched before any is used.
+        uint32_t bk[9], cn[9], st[9];
+#pragma unroll
+        for (uint32_t o = 0; o < 9; ++o) {
+            bk[o] = hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz, k.cellMask);
+            cn[o] = counts[bk[o]];
+            st[o] = bucketStart[bk[o]];
+        }
+#pragma unroll
+        for (uint32_t o = 0; o < 9; ++o) {
+            if (dedupe && seen_before(cx, cy, cz, (uint32_t)(dz + 1) * 9u + o, bk[o], k.cellMask)) continue;
+            uint32_t cnt = cn[o];
+            if (cnt == 0) continue;
+            const uint32_t lo = st[o];
+            if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+            for (uint32_t j = 0; j < cnt; j += 4) {
+                // Four candidates in flight, then consumed in slot order.
+                float qx[4], qy[4], qz[4];
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    const uint32_t jj = j + e < cnt ? j + e : cnt - 1u;
+                    const float* q = reinterpret_cast<const float*>(pos + lo + jj);
+                    qx[e] = q[0];
+                    qy[e] = q[1];
+                    qz[e] = q[2];
+                }
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    const float ddx = px - qx[e], ddy = py - qy[e], ddz = pz - qz[e];
                     const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
                     // poly6 is 0 at r2 >= h2 and density + mass * 0 == density: skip.
-                    if (r2 < k.h2) {
+                    if (j + e < cnt && r2 < k.h2) {
                         const float d = k.h2 - r2;
                         density += k.mass * (((k.cPoly6 * d) * d) * d);
+                        if (listed < cap) {
+                            const uint32_t u = lo + j + e, lane4 = listed & 3u;
+                            group.x = lane4 == 0u ? u : group.x;
+                            group.y = lane4 == 1u ? u : group.y;
+                            group.z = lane4 == 2u ? u : group.z;
+                            group.w = lane4 == 3u ? u : group.w;
+                            if (lane4 == 3u) store_stream(nbrList + (size_t)(listed >> 2) * k.n + t, group);
+                        }
+                        ++listed;
                     }
                 }
             }
@@ -83,6 +137,11 @@ __global__ void __launch_bounds__(kBlock)
     density = density < 1e-6f ? 1e-6f : density;
     reinterpret_cast<float*>(pos + t)[3] = density;
     reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
+    if (nbrCount) {
+        if (listed <= cap && (listed & 3u))
+            store_stream(nbrList + (size_t)(listed >> 2) * k.n + t, group);
+        nbrCount[t] = listed <= cap ? listed : cap + 1u;
+    }
 }
 
 // ---- boundary pieces (src/physics/force_field.h:31-70, collider_shapes.h:33-131) ----
@@ -249,11 +308,48 @@ __device__ void field_one(const SphForceField& f, const float p[3], float out[3]
     }
 }
 
+// Pressure + viscosity contribution of neighbour slot u to particle (p, v, pressure),
+// accumulated into f in the reference's order (fluid_sph.cpp:244-254).
+__device__ __forceinline__ void pair_force_from(const StepConsts& k, const float4 q, const float4 ov, float px,
+                                                float py, float pz, float vx, float vy, float vz, float pressure,
+                                                float f[3]) {
+    const float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
+    const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+    // r2 >= r2Cut <=> sqrtf(r2) >= h (both correctly rounded, monotone): the
+    // reference's `distance >= h` test without the square root.
+    if (r2 >= k.r2Cut) return;
+    const float distance = sqrtf(r2);
+    if (distance >= k.h || distance <= 1e-6f) return;
+    const float otherDensity = q.w < 1e-6f ? 1e-6f : q.w;
+    // spikyGradient (fluid_sph.cpp:33-39): ((c d) d) (delta / length)
+    const float d = k.h - distance;
+    const float g = (k.cSpiky * d) * d;
+    const float gx = g * (ddx / distance), gy = g * (ddy / distance), gz = g * (ddz / distance);
+    // -mass (p_i + p_j) / (2 rho_j) * grad (fluid_sph.cpp:251-252)
+    const float sp = (k.negMass * (pressure + ov.w)) / (2.0f * otherDensity);
+    f[0] += sp * gx;
+    f[1] += sp * gy;
+    f[2] += sp * gz;
+    // viscosity mass (v_j - v_i) / rho_j * laplacian (fluid_sph.cpp:253-254)
+    const float lap = k.cVisc * (k.h - distance);
+    f[0] += ((k.viscMass * (ov.x - vx)) / otherDensity) * lap;
+    f[1] += ((k.viscMass * (ov.y - vy)) / otherDensity) * lap;
+    f[2] += ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
+}
+
+__device__ __forceinline__ void pair_force(const StepConsts& k, const float4* __restrict__ pos,
+                                           const float4* __restrict__ vel, uint32_t u, float px, float py, float pz,
+                                           float vx, float vy, float vz, float pressure, float f[3]) {
+    pair_force_from(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
+}
+
 __global__ void __launch_bounds__(kBlock)
     k_force(StepConsts k, BoundaryConsts bnd, const float4* __restrict__ pos, const float4* __restrict__ vel,
-            const uint32_t* __restrict__ starts, float4* __restrict__ posOut, float4* __restrict__ velOut,
+            const uint32_t* __restrict__ bucketStart, const uint32_t* __restrict__ counts,
+            const uint4* __restrict__ nbrList,
+            const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut, float4* __restrict__ velOut,
             float4* __restrict__ forceOut) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= k.n) return;
     const float4 self = pos[t];
     const float4 selfVel = vel[t];
@@ -261,51 +357,53 @@ __global__ void __launch_bounds__(kBlock)
     const float density = self.w < 1e-6f ? 1e-6f : self.w;
     float vx = selfVel.x, vy = selfVel.y, vz = selfVel.z;
     const float pressure = selfVel.w;
-    const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
-    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
-    const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
-
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    uint32_t o = 0;
-    for (int32_t dz = -1; dz <= 1; ++dz) {
-        for (int32_t dy = -1; dy <= 1; ++dy) {
-            for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
-                const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
-                if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
-                const uint32_t lo = starts[b];
-                uint32_t cnt = starts[b + 1] - lo;
-                if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
-                for (uint32_t j = 0; j < cnt; ++j) {
-                    const uint32_t u = lo + j;
-                    if (u == t) continue;
-                    const float4 q = pos[u];
-                    const float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
-                    const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                    // r2 >= r2Cut <=> sqrtf(r2) >= h (both correctly rounded, monotone):
-                    // the reference's `distance >= h` test without the square root.
-                    if (r2 >= k.r2Cut) continue;
-                    const float distance = sqrtf(r2);
-                    if (distance >= k.h || distance <= 1e-6f) continue;
-                    const float otherDensity = q.w < 1e-6f ? 1e-6f : q.w;
-                    const float4 ov = vel[u];
-                    // spikyGradient (fluid_sph.cpp:33-39): ((c d) d) (delta / length)
-                    const float d = k.h - distance;
-                    const float g = (k.cSpiky * d) * d;
-                    const float gx = g * (ddx / distance), gy = g * (ddy / distance), gz = g * (ddz / distance);
-                    // -mass (p_i + p_j) / (2 rho_j) * grad (fluid_sph.cpp:251-252)
-                    const float sp = (k.negMass * (pressure + ov.w)) / (2.0f * otherDensity);
-                    fx += sp * gx;
-                    fy += sp * gy;
-                    fz += sp * gz;
-                    // viscosity mass (v_j - v_i) / rho_j * laplacian (fluid_sph.cpp:253-254)
-                    const float lap = k.cVisc * (k.h - distance);
-                    fx += ((k.viscMass * (ov.x - vx)) / otherDensity) * lap;
-                    fy += ((k.viscMass * (ov.y - vy)) / otherDensity) * lap;
-                    fz += ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    const uint32_t listed = nbrCount ? nbrCount[t] : k.listCap + 1u;
+    if (listed <= k.listCap) {
+        // The in-range candidates recorded by the density pass, in visiting order.
+        for (uint32_t e = 0; e < listed; e += 4) {
+            const uint4 g = load_stream(nbrList + (size_t)(e >> 2) * k.n + t);
+            const uint32_t us[4] = {g.x, g.y, g.z, g.w};
+            // All four neighbours' state in flight before the first is consumed; slots
+            // past the end of the list re-read the last valid one and are not used.
+            float4 q[4], ov[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint32_t u = e + r < listed ? us[r] : us[0];
+                q[r] = pos[u];
+                ov[r] = vel[u];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                if (e + r < listed && us[r] != t)
+                    pair_force_from(k, q[r], ov[r], px, py, pz, vx, vy, vz, pressure, f);
+            }
+        }
+    } else {
+        // No list (or it overflowed): walk the 27 buckets as the reference does.
+        const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
+        const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+        const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
+        uint32_t o = 0;
+        for (int32_t dz = -1; dz <= 1; ++dz) {
+            for (int32_t dy = -1; dy <= 1; ++dy) {
+                for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
+                    const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
+                    if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
+                    uint32_t cnt = counts[b];
+                    if (cnt == 0) continue;
+                    const uint32_t lo = bucketStart[b];
+                    if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const uint32_t u = lo + j;
+                        if (u == t) continue;
+                        pair_force(k, pos, vel, u, px, py, pz, vx, vy, vz, pressure, f);
+                    }
                 }
             }
         }
     }
+    const float fx = f[0], fy = f[1], fz = f[2];
     if (forceOut) forceOut[t] = make_float4(fx, fy, fz, 0.0f);
 
     // acceleration = force / density + gravity + fields (fluid_sph.cpp:260-261)
@@ -357,16 +455,20 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
-hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* starts, hipStream_t s) {
+hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* bucketStart, const uint32_t* counts,
+                          uint4* nbrList, uint32_t* nbrCount, hipStream_t s) {
     if (k.n == 0) return hipSuccess;
-    k_density<<<blocks_for(k.n), kBlock, 0, s>>>(k, pos, vel, starts);
+    k_density<<<blocks_for(k.n), kBlock, 0, s>>>(k, pos, vel, bucketStart, counts, nbrList, nbrCount);
     return hipGetLastError();
 }
 
 hipError_t launch_force(const StepConsts& k, const BoundaryConsts& b, const float4* pos, const float4* vel,
-                        const uint32_t* starts, float4* posOut, float4* velOut, float4* forceOut, hipStream_t s) {
+                        const uint32_t* bucketStart, const uint32_t* counts, const uint4* nbrList,
+                        const uint32_t* nbrCount, float4* posOut,
+                        float4* velOut, float4* forceOut, hipStream_t s) {
     if (k.n == 0) return hipSuccess;
-    k_force<<<blocks_for(k.n), kBlock, 0, s>>>(k, b, pos, vel, starts, posOut, velOut, forceOut);
+    k_force<<<blocks_for(k.n), kBlock, 0, s>>>(k, b, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut,
+                                               forceOut);
     return hipGetLastError();
 }
 

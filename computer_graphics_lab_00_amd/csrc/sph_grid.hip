@@ -2,17 +2,27 @@
 //
 // Replaces FluidGrid::build (src/physics/fluid_sph.cpp:160-188) and the GPU's
 // shaders/fluid_grid.comp:9-20. The reference inserts particles serially in ascending
-// index into 32-slot buckets; its GPU twin uses atomics and so loses that order. Here:
-//   1. keys:   key = spatialHash(spatialCell(p, h)) per particle; a device atomic
-//              hands out a provisional slot and counts the bucket (order-free integers)
-//   2. scan:   exclusive scan of the counts -> bucket starts (three passes, LDS + wave
-//              shuffles)
-//   3. scatter:(particle id, source slot) pairs land in their bucket's range
-//   4. rank:   each pair's final slot is its rank by particle id inside its bucket, so
-//              every bucket ends up in ascending index order whatever the atomics did;
-//              the particle's pos / vel / id are copied to that slot (the reorder)
-// The result is the reference's bucket order bit for bit, with no 32-slot cap on the
-// storage (the cap is applied by the neighbour loops in reference mode).
+// index into hash buckets of 32 slots; its GPU twin uses atomics and so loses that order.
+//
+// What the neighbour loops need is, for every hash bucket, its members in ascending
+// particle index. Where the buckets sit in memory is free, so they are laid out by
+// space: every bucket gets a representative spatial bin (the smallest bin of the cells
+// it holds; bins are 4x4x4 tiles of cells in Morton order inside a tile), particles are
+// counting-sorted by their bucket's bin, and inside a bin ordered by (bucket, index).
+// Each bucket therefore stays contiguous and index-ascending, and neighbouring cells
+// sit next to each other in HBM, which is what the 27-bucket gathers read.
+//   1. clear:   counts[C] = 0, rep[C] = ~0, binCount[bins] = 0
+//   2. keys:    key = spatialHash(spatialCell(p, h)); counts[key] += 1; rep[key] min= bin
+//               (wave-level run aggregation: the state is already spatially sorted, so
+//               neighbouring lanes mostly share a bucket)
+//   3. binslot: bin = rep[key]; a provisional slot in the bin by atomic
+//   4. scan:    exclusive scan of binCount -> binStart (three passes, LDS + shuffles)
+//   5. scatter: (key, id, source) lands in its bin's range
+//   6. rank:    final slot = bin start + rank of (key, id) among the bin's members; the
+//               particle's pos / vel / id are copied there, and the first member of
+//               each bucket records bucketStart[key]
+// The atomics only ever produce counts and provisional slots; every final position is
+// a pure function of the keys and indices, so the result is deterministic.
 #include "sph_device.h"
 
 namespace sphdev {
@@ -29,10 +39,45 @@ __device__ inline int32_t cell_coord(float x, float h) {
     return (int32_t)floorf(x / h);
 }
 
+__device__ inline uint32_t spread2(uint32_t v) { return (v & 1u) | ((v & 2u) << 2); }
+
+// Spatial bin of a cell: clamp into the bin grid, 4x4x4 tiles in x-fastest order,
+// Morton order of the 64 cells inside a tile.
+__device__ inline uint32_t bin_of(const BinGrid& g, int32_t cx, int32_t cy, int32_t cz) {
+    const int32_t c[3] = {cx, cy, cz};
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int64_t d = (int64_t)c[a] - g.origin[a];
+        d = d < 0 ? 0 : d;
+        d >>= g.shift;
+        q[a] = d >= (int64_t)g.dims[a] ? g.dims[a] - 1u : (uint32_t)d;
+    }
+    const uint32_t tile = ((q[2] >> 2) * g.tiles[1] + (q[1] >> 2)) * g.tiles[0] + (q[0] >> 2);
+    const uint32_t m = spread2(q[0] & 3u) | (spread2(q[1] & 3u) << 1) | (spread2(q[2] & 3u) << 2);
+    return tile * 64u + m;
+}
+
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Bit mask of lanes strictly below / at or below this lane.
+__device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ull : ((2ull << lane) - 1ull); }
+
 __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, uint32_t count) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t stride = gridDim.x * kBlock;
     for (; i < count; i += stride) dst[i] = value;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_grid_clear(uint32_t* __restrict__ counts, uint32_t* __restrict__ rep, uint32_t cells,
+                 uint32_t* __restrict__ binCount, uint32_t bins) {
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride) {
+        counts[i] = 0u;
+        rep[i] = 0xFFFFFFFFu;
+    }
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < bins; i += stride) binCount[i] = 0u;
 }
 
 __global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ ids, uint32_t n) {
@@ -41,14 +86,58 @@ __global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ ids, uin
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_keys(StepConsts k, const float4* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ slots,
-           uint32_t* __restrict__ counts) {
-    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= k.n) return;
-    float4 p = pos[s];
-    uint32_t key = hash_cell(cell_coord(p.x, k.h), cell_coord(p.y, k.h), cell_coord(p.z, k.h), k.cellMask);
-    keys[s] = key;
-    slots[s] = atomicAdd(&counts[key], 1u);
+    k_keys(StepConsts k, BinGrid g, const float4* __restrict__ pos, uint32_t* __restrict__ keys,
+           uint32_t* __restrict__ counts, uint32_t* __restrict__ rep) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const bool active = s < k.n;
+    const uint32_t lane = lane_id();
+    uint32_t key = 0xFFFFFFFFu, bin = 0xFFFFFFFFu;
+    if (active) {
+        const float4 p = pos[s];
+        const int32_t cx = cell_coord(p.x, k.h), cy = cell_coord(p.y, k.h), cz = cell_coord(p.z, k.h);
+        key = hash_cell(cx, cy, cz, k.cellMask);
+        bin = bin_of(g, cx, cy, cz);
+        keys[s] = key;
+    }
+    // Runs of equal keys across consecutive lanes add their length with one atomic.
+    const uint32_t prevKey = __shfl_up(key, 1, 64);
+    const uint32_t prevBin = __shfl_up(bin, 1, 64);
+    const bool head = active && (lane == 0u || prevKey != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t live = __ballot(active);
+    if (head) {
+        const uint64_t above = heads & ~lanes_upto(lane);
+        const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : (uint32_t)__popcll(live);
+        atomicAdd(&counts[key], end - lane);
+    }
+    if (active && (head || prevBin != bin)) atomicMin(&rep[key], bin);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_bin_slots(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rep,
+                uint32_t* __restrict__ binCount, uint32_t* __restrict__ binOf, uint32_t* __restrict__ slots) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const bool active = s < n;
+    const uint32_t lane = lane_id();
+    const uint32_t bin = active ? rep[keys[s]] : 0xFFFFFFFFu;
+    const uint32_t prevBin = __shfl_up(bin, 1, 64);
+    const bool head = active && (lane == 0u || prevBin != bin);
+    const uint64_t heads = __ballot(head);
+    const uint64_t live = __ballot(active);
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t above = heads & ~lanes_upto(lane);
+        const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : (uint32_t)__popcll(live);
+        base = atomicAdd(&binCount[bin], end - lane);
+    }
+    // Every lane takes its run head's base plus its distance from the head.
+    const uint64_t mine = heads & lanes_upto(lane);
+    const uint32_t headLane = mine ? 63u - (uint32_t)__clzll((unsigned long long)mine) : lane;
+    base = __shfl(base, headLane, 64);
+    if (active) {
+        binOf[s] = bin;
+        slots[s] = base + (lane - headLane);
+    }
 }
 
 // Inclusive scan across one wave64 with shuffles.
@@ -132,7 +221,7 @@ __global__ void __launch_bounds__(1024) k_scan_totals(uint32_t* __restrict__ sum
     }
 }
 
-// Pass 3: add each tile's offset; the element after the last bucket gets the total.
+// Pass 3: add each tile's offset; the element after the last one gets the total.
 __global__ void __launch_bounds__(kBlock)
     k_scan_add(uint32_t* __restrict__ data, uint32_t count, uint32_t total, const uint32_t* __restrict__ sums) {
     const uint32_t add = sums[blockIdx.x];
@@ -145,32 +234,43 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_scatter(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ slots,
-              const uint32_t* __restrict__ ids, const uint32_t* __restrict__ starts, uint2* __restrict__ pairs) {
-    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    k_scatter(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ binOf,
+              const uint32_t* __restrict__ slots, const uint32_t* __restrict__ ids,
+              const uint32_t* __restrict__ binStart, uint2* __restrict__ pairKI, uint32_t* __restrict__ pairSrc) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    uint32_t dst = starts[keys[s]] + slots[s];
-    pairs[dst] = make_uint2(ids[s], s);
+    const uint32_t dst = binStart[binOf[s]] + slots[s];
+    pairKI[dst] = make_uint2(keys[s], ids[s]);
+    pairSrc[dst] = s;
 }
 
-// Final slot = bucket start + number of bucket members with a smaller particle id.
+// Final slot = bin start + number of bin members before this one in (key, id) order.
 __global__ void __launch_bounds__(kBlock)
-    k_rank_reorder(uint32_t n, const uint2* __restrict__ pairs, const uint32_t* __restrict__ keys,
-                   const uint32_t* __restrict__ starts, const float4* __restrict__ pos,
-                   const float4* __restrict__ vel, float4* __restrict__ posSorted, float4* __restrict__ velSorted,
-                   uint32_t* __restrict__ idsSorted, uint32_t* __restrict__ keysSorted) {
-    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    k_rank_reorder(uint32_t n, const uint2* __restrict__ pairKI, const uint32_t* __restrict__ pairSrc,
+                   const uint32_t* __restrict__ binOf, const uint32_t* __restrict__ binStart,
+                   const float4* __restrict__ pos, const float4* __restrict__ vel, float4* __restrict__ posSorted,
+                   float4* __restrict__ velSorted, uint32_t* __restrict__ idsSorted,
+                   uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
-    uint2 pr = pairs[t];
-    uint32_t key = keys[pr.y];
-    uint32_t lo = starts[key], hi = starts[key + 1];
+    const uint2 me = pairKI[t];
+    const uint32_t src = pairSrc[t];
+    const uint32_t bin = binOf[src];
+    const uint32_t lo = binStart[bin], hi = binStart[bin + 1];
     uint32_t rank = 0;
-    for (uint32_t u = lo; u < hi; ++u) rank += pairs[u].x < pr.x ? 1u : 0u;
-    uint32_t dst = lo + rank;
-    posSorted[dst] = pos[pr.y];
-    velSorted[dst] = vel[pr.y];
-    idsSorted[dst] = pr.x;
-    keysSorted[dst] = key;
+    bool first = true;
+    for (uint32_t u = lo; u < hi; ++u) {
+        const uint2 o = pairKI[u];
+        const bool sameKey = o.x == me.x;
+        rank += (o.x < me.x || (sameKey && o.y < me.y)) ? 1u : 0u;
+        if (sameKey && o.y < me.y) first = false;
+    }
+    const uint32_t dst = lo + rank;
+    posSorted[dst] = pos[src];
+    velSorted[dst] = vel[src];
+    idsSorted[dst] = me.y;
+    keysSorted[dst] = me.x;
+    if (first) bucketStart[me.x] = dst;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -217,6 +317,13 @@ __global__ void __launch_bounds__(kBlock)
     ids[i] = i;
 }
 
+__global__ void __launch_bounds__(kBlock)
+    k_calibration_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * kBlock;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
@@ -235,8 +342,21 @@ hipError_t launch_iota(uint32_t* ids, uint32_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_keys(const StepConsts& k, const GridBuffers& g, hipStream_t s) {
-    k_keys<<<blocks_for(k.n), kBlock, 0, s>>>(k, g.pos, g.keys, g.slots, g.starts);
+hipError_t launch_grid_clear(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s) {
+    const uint32_t cells = k.cellMask + 1u;
+    uint32_t blocks = blocks_for(cells > g.bins ? cells : g.bins);
+    if (blocks > 4096u) blocks = 4096u;
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, cells, b.binStart, g.bins);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s) {
+    k_keys<<<blocks_for(k.n), kBlock, 0, s>>>(k, g, b.pos, b.keys, b.counts, b.rep);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_slots(const StepConsts& k, const GridBuffers& b, hipStream_t s) {
+    k_bin_slots<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, b.keys, b.rep, b.binStart, b.binOf, b.slots);
     return hipGetLastError();
 }
 
@@ -248,14 +368,16 @@ hipError_t launch_scan(uint32_t* data, uint32_t count, uint32_t total, uint32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const StepConsts& k, const GridBuffers& g, hipStream_t s) {
-    k_scatter<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, g.keys, g.slots, g.ids, g.starts, g.pairs);
+hipError_t launch_scatter(const StepConsts& k, const GridBuffers& b, hipStream_t s) {
+    k_scatter<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, b.keys, b.binOf, b.slots, b.ids, b.binStart, b.pairKI,
+                                                 b.pairSrc);
     return hipGetLastError();
 }
 
-hipError_t launch_rank_reorder(const StepConsts& k, const GridBuffers& g, hipStream_t s) {
-    k_rank_reorder<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, g.pairs, g.keys, g.starts, g.pos, g.vel, g.posSorted,
-                                                      g.velSorted, g.idsSorted, g.keysSorted);
+hipError_t launch_rank_reorder(const StepConsts& k, const GridBuffers& b, hipStream_t s) {
+    k_rank_reorder<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, b.pairKI, b.pairSrc, b.binOf, b.binStart, b.pos, b.vel,
+                                                      b.posSorted, b.velSorted, b.idsSorted, b.keysSorted,
+                                                      b.bucketStart);
     return hipGetLastError();
 }
 
@@ -287,6 +409,11 @@ hipError_t launch_emit(const SphParams& p, uint32_t count, float4* pos, float4* 
     e.cy = c[1];
     e.count = count;
     k_emit<<<blocks_for(count), kBlock, 0, s>>>(e, pos, vel, prev, ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
+    k_calibration_copy<<<8192, kBlock, 0, s>>>(src, dst, n);
     return hipGetLastError();
 }
 

@@ -161,10 +161,11 @@ class FluidSolver:
         return cell_of, counts, slots
 
     def sorted_layout(self, cell_count: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(keys, ids) per device slot and bucketStarts[cellCount] (0xFFFFFFFF if empty)."""
         n = self.particle_count()
         keys = np.empty(n, np.uint32)
         ids = np.empty(n, np.uint32)
-        starts = np.empty(cell_count + 1, np.uint32)
+        starts = np.empty(cell_count, np.uint32)
         check(self._lib.sph_read_sorted(self._h, _uptr(keys), _uptr(ids), _uptr(starts)))
         return keys, ids, starts
 

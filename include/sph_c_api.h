@@ -154,6 +154,8 @@ typedef struct SphSolverConfig {
 
 #define SPH_CFG_KEEP_FORCES 1u  /* keep each particle's pressure+viscosity force of the last substep */
 #define SPH_CFG_NO_GRAPH 2u     /* launch substeps eagerly instead of replaying a captured hipGraph */
+#define SPH_CFG_BUCKET_WALK 4u  /* force pass walks the 27 buckets instead of the density pass's
+                                 * neighbour list (saves 256 B of device memory per particle) */
 
 /* ---- stateless helpers ---- */
 
@@ -235,9 +237,12 @@ int sph_load_state(SphSolver* solver, const float* pos4, const float* vel4, uint
  * bucket valid (the rest are 0xFFFFFFFF). Any pointer may be NULL. Synchronous. */
 int sph_read_grid(SphSolver* solver, uint32_t* cellOf, uint32_t* cellCounts, uint32_t* cellParticles);
 
-/* The bucket-sorted layout behind the grid: keys[count] and ids[count] in sorted order,
- * starts[cellCount + 1]. Any pointer may be NULL. Synchronous. */
-int sph_read_sorted(SphSolver* solver, uint32_t* keys, uint32_t* ids, uint32_t* starts);
+/* The device layout behind the grid: keys[count] and ids[count] per slot in memory
+ * order, and bucketStarts[cellCount], the first slot of each bucket (0xFFFFFFFF for empty
+ * buckets). Every bucket occupies bucketStarts[b] .. + cellCounts[b] in ascending
+ * particle index; buckets are ordered by space, not by key. Any pointer may be NULL.
+ * Synchronous. */
+int sph_read_sorted(SphSolver* solver, uint32_t* keys, uint32_t* ids, uint32_t* bucketStarts);
 
 /* Pressure + viscosity force of the last substep, per particle in index order, as
  * (fx, fy, fz, 0) — the `force` accumulator of src/physics/fluid_sph.cpp:233-258, which

@@ -54,9 +54,9 @@ def norm_err(x, ref):
     return float(np.max(np.linalg.norm(x - ref, axis=1)) / max(np.max(np.linalg.norm(ref, axis=1)), 1e-30))
 
 
-def one_substep_both(p, pos, vel, dt, oracle_lib, mode=0, eos=0, max_cells=None):
+def one_substep_both(p, pos, vel, dt, oracle_lib, mode=0, eos=0, max_cells=None, flags=0):
     n = pos.shape[0]
-    gpu = fluid.FluidSolver(n, max_cells or max(p.cellCount, 1024), keep_forces=True)
+    gpu = fluid.FluidSolver(n, max_cells or max(p.cellCount, 1024), keep_forces=True, flags=flags)
     gpu.set_mode(mode)
     gpu.set_eos(eos)
     gpu.load_state(pos, vel)
@@ -136,18 +136,30 @@ def test_golden_fixture(name):
         assert_bit_equal(gpu.forces(), ref["force"], "fixture force")
 
 
-@pytest.mark.parametrize("cells", [32768, 1024])
-def test_s16k_one_substep(cells, oracle_lib):
-    say(f"case: s16k one substep, cellCount {cells}")
+@pytest.mark.parametrize("cells,flags", [(32768, 0), (1024, 0), (32768, capi.CFG_BUCKET_WALK),
+                                         (1024, capi.CFG_BUCKET_WALK)])
+def test_s16k_one_substep(cells, flags, oracle_lib):
+    say(f"case: s16k one substep, cellCount {cells}, flags {flags}")
     w = fluid.s16k(cells)
     p = w.params()
     pos, vel = w.initial_state(p)
     rng = np.random.default_rng(1)
     vel[:, :3] = rng.normal(0, 0.5, (w.count, 3)).astype(np.float32)
     dt = 1.0 / 600.0 / fluid.fluid_substeps(p, 1.0 / 600.0)
-    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib, flags=flags)
     compare_substep(gpu, ora, p, f"s16k C={cells}")
     fp64_report(p, pos, vel, dt, gpu, oracle_lib, f"s16k C={cells}")
+    gpu.close()
+
+
+def test_neighbour_list_overflow_falls_back(oracle_lib):
+    """cellCount = 1 and 40 particles: every particle has 27 x 32 in-range entries, far
+    past the list capacity, so the force pass walks the buckets; still bit-exact."""
+    say("case: neighbour-list overflow")
+    ref = np.load(G.HERE / "sph_n40_c1.npz")
+    p, _, _, _ = G.case_inputs("n40_c1")
+    gpu, ora = one_substep_both(p, ref["pos0"], ref["vel0"], float(ref["dt"]), oracle_lib)
+    compare_substep(gpu, ora, p, "list overflow")
     gpu.close()
 
 
@@ -292,14 +304,15 @@ def test_gpu_runs_are_bitwise_deterministic():
     assert_bit_equal(outs[0][1], outs[1][1], "run-to-run velocities")
 
 
-def test_m1_one_substep(oracle_lib):
+@pytest.mark.parametrize("flags", [0, capi.CFG_BUCKET_WALK])
+def test_m1_one_substep(flags, oracle_lib):
     """1,048,576 particles, 2^21 buckets: grid and one substep bit-exact; FP64 error."""
-    say("case: m1 one substep, cellCount 2^21")
+    say(f"case: m1 one substep, cellCount 2^21, flags {flags}")
     w = fluid.m1()
     p = w.params()
     pos, vel = w.initial_state(p)
     dt = 1.0 / 600.0 / 8
-    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib, flags=flags)
     compare_substep(gpu, ora, p, "m1 C=2^21")
     fp64_report(p, pos, vel, dt, gpu, oracle_lib, "m1 C=2^21")
     gpu.close()
@@ -333,10 +346,17 @@ def test_m1_full_run_properties():
         for _ in range(5):
             gpu.step(p, 1.0 / 600.0)
         keys, ids, starts = gpu.sorted_layout(p.cellCount)
-        assert starts[-1] == w.count and np.all(np.diff(starts.astype(np.int64)) >= 0)
+        _, counts, _ = gpu.grid(p.cellCount)
+        assert counts.sum() == w.count
         assert np.array_equal(np.sort(ids), np.arange(w.count, dtype=np.uint32))
-        assert np.all(np.diff(keys.astype(np.int64)) >= 0)
-        same = keys[1:] == keys[:-1]
+        # every bucket is one contiguous run of its key, ascending in particle index
+        occupied = np.flatnonzero(counts)
+        assert np.all(starts[occupied] != 0xFFFFFFFF)
+        assert np.all(starts[counts == 0] == 0xFFFFFFFF)
+        heads = np.r_[True, keys[1:] != keys[:-1]]
+        assert heads.sum() == len(occupied)                 # one run per bucket
+        assert np.array_equal(np.sort(np.flatnonzero(heads)), np.sort(starts[occupied]))
+        same = ~heads[1:]
         assert np.all(ids[1:][same] > ids[:-1][same])
         m, ke, pe = gpu.conservation(p)
         assert m == m0
@@ -382,3 +402,14 @@ def test_keep_rendered_and_previous():
         assert np.array_equal(gpu.previous_particles()[:, :3], first[:, :3])
         gpu.keep_rendered()
         assert np.array_equal(gpu.previous_particles(), gpu.particles())
+
+
+def test_cxx_adapter_runs_on_gpu(tmp_path):
+    """The header-only C++ adapter drives emit / step / particles on the GPU."""
+    say("case: c++ adapter")
+    from tests.test_host import _build_adapter_example
+    import subprocess
+    exe = _build_adapter_example(tmp_path)
+    res = subprocess.run([str(exe), "30"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "particles 512" in res.stdout

@@ -136,3 +136,22 @@ def test_workload_parameters():
     assert p.smoothingRadius == pytest.approx(0.0156)
     assert fluid.fluid_substeps(p, 1 / 600) == 8
     assert fluid.fluid_substeps(fluid.s16k().params(), 1 / 600) == 5
+
+
+def _build_adapter_example(tmp_path):
+    exe = tmp_path / "adapter_example"
+    lib_dir = capi.LIB_PATH.parent
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cxx" / "adapter_example.cpp"),
+                    f"-L{lib_dir}", "-lsph_hip", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cxx_adapter_compiles_and_reports_no_device(sph, tmp_path):
+    """include/sph_fluid.hpp compiles with the host compiler alone and links against the
+    library; without a GPU the adapter surfaces SPH_ERR_NO_DEVICE as an exception."""
+    import os
+    exe = _build_adapter_example(tmp_path)
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible; the GPU suite runs the example for real")
+    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert res.returncode == 2, res.stdout + res.stderr
