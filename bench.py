@@ -51,7 +51,72 @@ def parse():
                     help="force pass walks the 27 buckets instead of the density pass's neighbour list")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one 256 MiB device copy kernel first (byte-counter calibration)")
+    ap.add_argument("--workload", default="dam_break_1m", choices=["dam_break_1m", "batch64"],
+                    help="dam_break_1m: the headline line; batch64: 64 independent scenes in one hipGraph")
+    ap.add_argument("--tune", action="store_true", help="batch64: re-run the autotuner instead of its cache")
     return ap.parse_args()
+
+
+TUNE_CACHE = ROOT / "computer_graphics_lab_00_amd" / "tuned" / "batch64_mi355x.json"
+
+
+def bench_batch64(args) -> None:
+    """64 independent scenes (2k-200k particles, log-uniform, seeded) advanced as one batch:
+    one launch per kernel per substep for all scenes, each step replayed from a hipGraph.
+    Compared with the same scenes stepped one after another on single-scene solvers."""
+    from computer_graphics_lab_00_amd import batch
+
+    specs = batch.batch64(2024)
+    if args.tune or TUNE_CACHE.exists():
+        print("[bench] autotune (cache: %s)" % TUNE_CACHE, file=sys.stderr, flush=True)
+        specs = batch.autotune(specs, TUNE_CACHE, retune=args.tune)
+    total = sum(s.count for s in specs)
+    with batch.SceneBatch(specs) as b:
+        b.load_initial()
+        for _ in range(args.warmup):
+            b.step(DT)
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.step(DT)
+        b.synchronize()
+        elapsed = time.perf_counter() - t0
+    print(f"[bench] batch of {len(specs)} scenes ({total} particles): {elapsed * 1e3 / args.steps:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    # The same scenes one after another, each in its own single-scene batch.
+    seq = 0.0
+    for s in specs:
+        with batch.SceneBatch([s]) as one:
+            one.load_initial()
+            for _ in range(2):
+                one.step(DT)
+            one.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                one.step(DT)
+            one.synchronize()
+            seq += time.perf_counter() - t0
+    value = total * args.steps / elapsed
+    line = {
+        "metric": "particle-steps/s, 64 independent scenes batched on one GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch64", "scenes": len(specs), "particles": total,
+                   "counts": [s.count for s in specs], "seed": 2024, "dt": DT,
+                   "tuned": TUNE_CACHE.exists()},
+        "sequential_ms_per_step": seq * 1e3 / args.steps,
+        "batched_speedup_vs_sequential": seq / elapsed,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
@@ -105,6 +170,9 @@ def main():
     from computer_graphics_lab_00_amd import capi, fluid
     if args.calibrate:
         capi.check(capi.load().sph_debug_copy(256 << 20))
+    if args.workload == "batch64":
+        bench_batch64(args)
+        return
 
     w = fluid.m1(args.cells)
     p = w.params()

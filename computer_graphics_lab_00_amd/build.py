@@ -25,6 +25,7 @@ SOURCES = [
     "sph_host.cpp",
     "sph_grid.hip",
     "sph_forces.hip",
+    "sph_engine.hip",
     "sph_api.hip",
 ]
 
@@ -44,7 +45,6 @@ DEVICE_FLAGS = [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
 ]
 
 

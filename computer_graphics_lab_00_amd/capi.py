@@ -130,6 +130,10 @@ class SphSolverConfig(C.Structure):
     ]
 
 
+class SphSceneCapacity(C.Structure):
+    _fields_ = [("maxParticles", C.c_uint32), ("maxCellCount", C.c_uint32)]
+
+
 class SphError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -174,6 +178,21 @@ def _declare(lib):
         "sph_conservation": (C.c_int, [P, C.POINTER(SphParams), _DP]),
         "sph_profile_enable": (C.c_int, [P, C.c_int]),
         "sph_profile_read": (C.c_int, [P, C.POINTER(C.c_char_p), _DP, _UP, C.c_uint32]),
+        "sph_batch_create": (C.c_int, [C.POINTER(SphSceneCapacity), C.c_uint32, C.c_int32, C.c_uint32,
+                                       C.POINTER(P)]),
+        "sph_batch_destroy": (C.c_int, [P]),
+        "sph_batch_scene_count": (C.c_uint32, [P]),
+        "sph_batch_set_params": (C.c_int, [P, C.c_uint32, C.POINTER(SphParams)]),
+        "sph_batch_set_layout": (C.c_int, [P, C.c_uint32, _UP, C.c_uint32]),
+        "sph_batch_set_mode": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "sph_batch_emit": (C.c_int, [P, C.c_uint32, C.c_uint32]),
+        "sph_batch_load_state": (C.c_int, [P, C.c_uint32, _FP, _FP, C.c_uint32]),
+        "sph_batch_step": (C.c_int, [P, C.c_float]),
+        "sph_batch_synchronize": (C.c_int, [P]),
+        "sph_batch_particle_count": (C.c_uint32, [P, C.c_uint32]),
+        "sph_batch_read_particles": (C.c_int, [P, C.c_uint32, _FP, _FP]),
+        "sph_batch_profile_enable": (C.c_int, [P, C.c_int]),
+        "sph_batch_profile_read": (C.c_int, [P, C.POINTER(C.c_char_p), _DP, _UP, C.c_uint32]),
         "sph_debug_step_constants": (C.c_int, [C.POINTER(SphParams), C.c_float, _FP]),
         "sph_debug_copy": (C.c_int, [C.c_uint64]),
     }

@@ -1,14 +1,18 @@
-// sph_device.h — device-side types shared by the SPH kernels and the host driver.
+// sph_device.h — device-side types shared by the SPH kernels and the host engine.
 //
-// State layout in HBM (one solver):
-//   pos4[n]  float4 (x, y, z, density)     ping-pong pair
-//   vel4[n]  float4 (vx, vy, vz, pressure) ping-pong pair
-//   ids[n]   uint32 particle index of each slot, ping-pong pair
-//   counts[C]       members of each hash bucket (the reference's uncapped cellCounts)
-//   bucketStart[C]  first slot of each non-empty bucket
-// Slots are ordered by space (spatial bin of the bucket), then bucket, then particle
-// index. Every bucket is contiguous and ascending in index — the order of the reference's
-// serial insertion (src/physics/fluid_sph.cpp:181-187) — so its
+// One engine advances S independent scenes (S = 1 for a plain solver) with the same
+// kernels: every launch covers all scenes, and each workgroup finds its scene in a
+// per-block table, so a batch of uneven scenes costs one launch per kernel.
+//
+// State layout in HBM, global slot space = sum of scene capacities:
+//   pos4[slot]  float4 (x, y, z, density)     ping-pong pair
+//   vel4[slot]  float4 (vx, vy, vz, pressure) ping-pong pair
+//   ids[slot]   uint32 particle index within its scene, ping-pong pair
+//   counts[bucket], bucketStart[bucket]       hash buckets of all scenes, scene by scene
+// Scene s owns slots [offset_s, offset_s + capacity_s), its particles the first n_s.
+// Inside a scene slots are ordered by space (spatial bin of the bucket), then bucket,
+// then particle index. Every bucket is contiguous and ascending in index — the order of
+// the reference's serial insertion (src/physics/fluid_sph.cpp:181-187) — so its
 // `cellParticles[b * 32 + k]` is `ids[bucketStart[b] + k]` for k < min(counts[b], 32).
 #pragma once
 
@@ -24,11 +28,11 @@ constexpr uint32_t kScanPerThread = 8;    // elements each scan thread owns
 constexpr uint32_t kScanTile = kBlock * kScanPerThread;
 constexpr uint32_t kXcds = 8;             // MI355X: 8 XCDs, each with its own L2
 
-// Scalars of one substep, precomputed on the host exactly as the reference computes
-// them (src/physics/fluid_sph.cpp:24-46, :194-195, :264, :271-272) so device and
+// Scalars of one substep of one scene, precomputed on the host exactly as the reference
+// computes them (src/physics/fluid_sph.cpp:24-46, :194-195, :264, :271-272) so device and
 // oracle see identical values.
 struct StepConsts {
-    uint32_t n;
+    uint32_t n;             // particles in the scene
     uint32_t cellMask;      // cellCount - 1
     uint32_t mode;          // SphModeFlags
     uint32_t eos;           // SphEos
@@ -56,45 +60,67 @@ struct StepConsts {
     uint32_t listCap;       // neighbour-list slots per particle (0: no list)
 };
 
-// Spatial bins that order the buckets in memory: the container's cells (plus a margin),
-// coarsened by 2^shift per axis if needed, in 4x4x4 tiles.
+// Spatial bins that order a scene's buckets in memory: the container's cells (plus a
+// margin), coarsened by 2^shift per axis if needed, in tiles of 2^tileLog2 cells per
+// axis (64 bins per tile; Morton order inside a 4x4x4 tile, x-fastest otherwise).
 struct BinGrid {
     int32_t origin[3];      // cell coordinate of bin (0, 0, 0)
     uint32_t dims[3];       // bins per axis
-    uint32_t tiles[3];      // 4-bin tiles per axis
+    uint32_t tiles[3];      // tiles per axis
+    uint32_t tileLog2[3];   // sums to 6
     uint32_t shift;
     uint32_t bins;          // tiles product * 64
 };
 
-// Neighbour-list slots per particle: the in-range count of the reference's kernel
-// (h = 2 spacing) is about 33, 45 in compressed fluid; beyond the cap the force pass
-// falls back to walking the buckets.
-constexpr uint32_t kNeighbourCap = 64;
+struct SceneDesc {
+    StepConsts k;
+    BinGrid g;
+    uint32_t offset;        // first slot
+    uint32_t capacity;      // slots reserved
+    uint32_t cellOffset;    // first hash bucket
+    uint32_t binOffset;     // first spatial bin
+    uint32_t binCapacity;   // bins reserved; the filler bin binOffset + binCapacity holds capacity - n
+    uint32_t blockOffset;   // first workgroup of the scene in particle-parallel launches
+    uint32_t active;        // 0: this substep is past the scene's substep count; state passes through
+    uint32_t pad;
+};
 
-// Colliders and force fields travel in kernel arguments (at most 8 each).
+// Colliders and force fields of one scene (at most 8 each).
 struct BoundaryConsts {
     SphCollider colliders[SPH_MAX_COLLIDERS];
     SphForceField fields[SPH_MAX_FORCE_FIELDS];
+};
+
+// Everything a batched kernel needs to find its scene and the global arrays.
+struct Launch {
+    const SceneDesc* scenes;
+    const BoundaryConsts* bounds;
+    const uint16_t* blockScene;   // scene of each logical workgroup
+    uint32_t blocks;              // particle-parallel workgroups
+    uint32_t numScenes;
+    uint32_t total;               // slots (sum of capacities): neighbour-list stride
+    uint32_t totalCells;
+    uint32_t totalBins;           // sum of (binCapacity + 1)
 };
 
 struct GridBuffers {
     const float4* pos;      // state before the sort (previous layout)
     const float4* vel;
     const uint32_t* ids;
-    uint32_t* keys;         // hash key per pre-sort slot
-    uint32_t* counts;       // C
-    uint32_t* rep;          // C: smallest spatial bin among each bucket's cells
-    uint32_t* binStart;     // bins + 1: counts, then their exclusive scan
+    uint32_t* keys;         // global bucket per pre-sort slot
+    uint32_t* counts;       // totalCells
+    uint32_t* rep;          // totalCells: smallest global bin among each bucket's cells
+    uint32_t* binStart;     // totalBins + 1: counts, then their exclusive scan
     uint32_t* binOf;        // per pre-sort slot
     uint32_t* slots;        // provisional slot inside the bin (arbitrary order)
-    uint2* pairKI;          // (key, id) scattered by bin
+    uint2* pairKI;          // (global bucket, id) scattered by bin
     uint32_t* pairSrc;      // pre-sort slot of each scattered pair
     uint32_t* blockSums;
     float4* posSorted;
     float4* velSorted;
     uint32_t* idsSorted;
     uint32_t* keysSorted;
-    uint32_t* bucketStart;  // C
+    uint32_t* bucketStart;  // totalCells
 };
 
 // Logical block for a hardware workgroup id. Workgroups are dispatched round-robin over
@@ -106,27 +132,45 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t blocks) {
     return x * q + (x < r ? x : r) + k;
 }
 
-// launchers (sph_grid.hip)
-hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t s);
-hipError_t launch_grid_clear(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s);
-hipError_t launch_keys(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s);
-hipError_t launch_bin_slots(const StepConsts& k, const GridBuffers& b, hipStream_t s);
+// The scene, local index and global slot of this thread in a particle-parallel launch.
+struct Where {
+    const SceneDesc* d;
+    uint32_t local;
+    uint32_t slot;
+    bool valid;
+};
+
+__device__ __forceinline__ Where locate(const Launch& L) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const SceneDesc* d = L.scenes + L.blockScene[lb];
+    Where w;
+    w.d = d;
+    w.local = (lb - d->blockOffset) * kBlock + threadIdx.x;
+    w.valid = w.local < d->k.n;
+    w.slot = d->offset + w.local;
+    return w;
+}
+
+// launchers (sph_grid.hip); all batched launches use L.blocks workgroups
+hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t s);
+hipError_t launch_keys(const Launch& L, const GridBuffers& b, hipStream_t s);
+hipError_t launch_bin_slots(const Launch& L, const GridBuffers& b, hipStream_t s);
 hipError_t launch_scan(uint32_t* data, uint32_t count, uint32_t total, uint32_t* blockSums, hipStream_t s);
-hipError_t launch_scatter(const StepConsts& k, const GridBuffers& b, hipStream_t s);
-hipError_t launch_rank_reorder(const StepConsts& k, const GridBuffers& b, hipStream_t s);
+hipError_t launch_scatter(const Launch& L, const GridBuffers& b, hipStream_t s);
+hipError_t launch_rank_reorder(const Launch& L, const GridBuffers& b, hipStream_t s);
 hipError_t launch_unpermute(const float4* src, const uint32_t* ids, float4* dst, uint32_t n, hipStream_t s);
-hipError_t launch_gather_keys(const uint32_t* keysSorted, const uint32_t* ids, uint32_t* cellOf, uint32_t n,
-                              hipStream_t s);
+hipError_t launch_gather_keys(const uint32_t* keysSorted, const uint32_t* ids, uint32_t keyBase, uint32_t* cellOf,
+                              uint32_t n, hipStream_t s);
 hipError_t launch_emit(const SphParams& p, uint32_t count, float4* pos, float4* vel, float4* prev, uint32_t* ids,
                        hipStream_t s);
 hipError_t launch_iota(uint32_t* ids, uint32_t n, hipStream_t s);
 hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 
 // launchers (sph_forces.hip)
-hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* bucketStart,
+hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
                           const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, hipStream_t s);
-hipError_t launch_force(const StepConsts& k, const BoundaryConsts& b, const float4* pos, const float4* vel,
-                        const uint32_t* bucketStart, const uint32_t* counts, const uint4* nbrList,
-                        const uint32_t* nbrCount, float4* posOut, float4* velOut, float4* forceOut, hipStream_t s);
+hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
+                        const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
+                        float4* velOut, float4* forceOut, hipStream_t s);
 
 }  // namespace sphdev

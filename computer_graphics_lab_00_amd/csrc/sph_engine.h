@@ -1,0 +1,130 @@
+// sph_engine.h — the host engine behind both C-ABI handles: S independent scenes in one
+// set of device arrays, advanced together by batched launches on one HIP stream, with
+// the per-step launch sequence captured once as a hipGraph and replayed.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "sph_device.h"
+
+namespace sphhost {
+
+struct SceneHost {
+    uint32_t maxN = 0, maxC = 0, binCap = 0;
+    uint32_t offset = 0, cellOffset = 0, binOffset = 0;   // fixed at creation (capacity based)
+    uint32_t n = 0;
+    SphParams params{};
+    bool hasParams = false;
+    uint32_t mode = SPH_MODE_REFERENCE;
+    uint32_t eos = SPH_EOS_LINEAR;
+    uint32_t tileLog2[3] = {2, 2, 2};
+    bool list = true;
+};
+
+enum KernelClass : uint32_t {
+    KC_FILL = 0,
+    KC_KEYS,
+    KC_BINSLOT,
+    KC_SCAN,
+    KC_SCATTER,
+    KC_RANK,
+    KC_DENSITY,
+    KC_FORCE,
+    KC_COUNT
+};
+
+struct TimedLaunch {
+    uint32_t cls;
+    hipEvent_t begin;
+    hipEvent_t end;
+};
+
+constexpr uint32_t kMaxSubsteps = 8;
+
+class Engine {
+public:
+    ~Engine();
+    // capacities: {maxParticles, maxCellCount} per scene
+    int create(const std::vector<std::pair<uint32_t, uint32_t>>& capacities, int device, uint32_t flags);
+
+    int emit(uint32_t scene, const SphParams& p, uint32_t count);
+    int load_state(uint32_t scene, const float* pos4, const float* vel4, uint32_t count);
+    int set_params(uint32_t scene, const SphParams& p);
+    // Every scene advances by dt with its own substep rule (FluidSolver::step); when
+    // `singleSubstep` the scenes advance one substep of exactly dt.
+    int step(float dt, bool singleSubstep);
+    int synchronize();
+
+    int read_particles(uint32_t scene, float* pos4, float* vel4);
+    int read_previous(uint32_t scene, float* pos4);
+    int keep_rendered(uint32_t scene);
+    int read_grid(uint32_t scene, uint32_t* cellOf, uint32_t* cellCounts, uint32_t* cellParticles);
+    int read_sorted(uint32_t scene, uint32_t* keys, uint32_t* ids, uint32_t* bucketStarts);
+    int read_forces(uint32_t scene, float* force4);
+
+    int profile_read(const char** names, double* ms, uint32_t* launches, uint32_t capacity);
+
+    int set_device() const;
+
+    std::vector<SceneHost> sc;
+    int device = 0;
+    uint32_t flags = 0;
+    bool profiling = false;
+    hipStream_t stream = nullptr;
+
+private:
+    int ensure_plan(float dt, bool singleSubstep, uint32_t* maxSub);
+    int run_substep(uint32_t j);
+    int enqueue_step(uint32_t maxSub);
+    void invalidate();
+    sphdev::Launch launch_for(uint32_t j) const;
+
+    uint32_t totalN = 0, totalC = 0, totalBins = 0, maxBlocks = 0;
+    float4* pos[2] = {nullptr, nullptr};
+    float4* vel[2] = {nullptr, nullptr};
+    uint32_t* ids[2] = {nullptr, nullptr};
+    int pc = 0, ic = 0;
+    uint32_t *keys = nullptr, *counts = nullptr, *rep = nullptr, *bucketStart = nullptr;
+    uint32_t *binStart = nullptr, *binOf = nullptr, *slots = nullptr, *blockSums = nullptr, *keysSorted = nullptr;
+    uint2* pairKI = nullptr;
+    uint32_t* pairSrc = nullptr;
+    uint4* nbrList = nullptr;
+    uint32_t* nbrCount = nullptr;
+    float4 *prev = nullptr, *force = nullptr, *staging = nullptr;
+    uint32_t* stagingU = nullptr;
+    // plan: per-substep scene tables, boundaries, block -> scene
+    sphdev::SceneDesc* dTables = nullptr;
+    sphdev::BoundaryConsts* dBounds = nullptr;
+    uint16_t* dBlockScene = nullptr;
+    void* hostStage = nullptr;
+    size_t hostStageBytes = 0;
+    std::vector<uint8_t> planKey;
+    uint32_t planSub = 0, planBlocks = 0, planBins = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool hasGrid = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> eventPool;
+    double accMs[KC_COUNT] = {};
+    uint32_t accLaunches[KC_COUNT] = {};
+
+    friend struct Timer;
+    hipEvent_t take_event();
+    void record(uint32_t cls, hipEvent_t b, hipEvent_t e) { timed.push_back({cls, b, e}); }
+    void free_all();
+};
+
+// Host-side helpers shared with the batch and surrogate code.
+sphdev::StepConsts make_consts(const SceneHost& s, const SphParams& p, float dt, bool hasList);
+sphdev::BinGrid make_bins(const SceneHost& s, const SphParams& p);
+int hip_fail(hipError_t e, const char* what);
+
+}  // namespace sphhost
+
+#define SPH_HIP(call)                                                     \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return ::sphhost::hip_fail(e_, #call);      \
+    } while (0)

@@ -69,11 +69,13 @@ __device__ inline float eos_pressure(const StepConsts& k, float density) {
 // access. A particle with more than `listCap` entries stores count = listCap + 1 and the
 // force pass scans its buckets instead.
 __global__ void __launch_bounds__(kBlock)
-    k_density(StepConsts k, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
-              const uint32_t* __restrict__ counts,
-              uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount) {
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= k.n) return;
+    k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
+              const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount) {
+    const Where w = locate(L);
+    if (!w.valid || !w.d->active) return;
+    const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
+    const uint32_t t = w.slot;
+    const uint32_t cellBase = w.d->cellOffset;
     // Only xyz of any slot is read here; .w of this lane's own slot is written below
     // (the reference writes positions[i].w in place the same way, fluid_sph.cpp:197-199).
     const float* self = reinterpret_cast<const float*>(pos + t);
@@ -91,8 +93,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (uint32_t o = 0; o < 9; ++o) {
             bk[o] = hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz, k.cellMask);
-            cn[o] = counts[bk[o]];
-            st[o] = bucketStart[bk[o]];
+            cn[o] = counts[cellBase + bk[o]];
+            st[o] = bucketStart[cellBase + bk[o]];
         }
 #pragma unroll
         for (uint32_t o = 0; o < 9; ++o) {
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(kBlock)
                             group.y = lane4 == 1u ? u : group.y;
                             group.z = lane4 == 2u ? u : group.z;
                             group.w = lane4 == 3u ? u : group.w;
-                            if (lane4 == 3u) store_stream(nbrList + (size_t)(listed >> 2) * k.n + t, group);
+                            if (lane4 == 3u) store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
                         }
                         ++listed;
                     }
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(kBlock)
     reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
     if (nbrCount) {
         if (listed <= cap && (listed & 3u))
-            store_stream(nbrList + (size_t)(listed >> 2) * k.n + t, group);
+            store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
         nbrCount[t] = listed <= cap ? listed : cap + 1u;
     }
 }
@@ -344,13 +346,21 @@ __device__ __forceinline__ void pair_force(const StepConsts& k, const float4* __
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_force(StepConsts k, BoundaryConsts bnd, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    k_force(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
             const uint32_t* __restrict__ bucketStart, const uint32_t* __restrict__ counts,
-            const uint4* __restrict__ nbrList,
-            const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut, float4* __restrict__ velOut,
-            float4* __restrict__ forceOut) {
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= k.n) return;
+            const uint4* __restrict__ nbrList, const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut,
+            float4* __restrict__ velOut, float4* __restrict__ forceOut) {
+    const Where w = locate(L);
+    if (!w.valid) return;
+    const uint32_t t = w.slot;
+    if (!w.d->active) {
+        // Past this scene's substep count: the state passes through unchanged.
+        posOut[t] = pos[t];
+        velOut[t] = vel[t];
+        return;
+    }
+    const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
+    const uint32_t cellBase = w.d->cellOffset;
     const float4 self = pos[t];
     const float4 selfVel = vel[t];
     float px = self.x, py = self.y, pz = self.z;
@@ -362,7 +372,7 @@ __global__ void __launch_bounds__(kBlock)
     if (listed <= k.listCap) {
         // The in-range candidates recorded by the density pass, in visiting order.
         for (uint32_t e = 0; e < listed; e += 4) {
-            const uint4 g = load_stream(nbrList + (size_t)(e >> 2) * k.n + t);
+            const uint4 g = load_stream(nbrList + (size_t)(e >> 2) * L.total + t);
             const uint32_t us[4] = {g.x, g.y, g.z, g.w};
             // All four neighbours' state in flight before the first is consumed; slots
             // past the end of the list re-read the last valid one and are not used.
@@ -390,9 +400,9 @@ __global__ void __launch_bounds__(kBlock)
                 for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
                     const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
                     if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
-                    uint32_t cnt = counts[b];
+                    uint32_t cnt = counts[cellBase + b];
                     if (cnt == 0) continue;
-                    const uint32_t lo = bucketStart[b];
+                    const uint32_t lo = bucketStart[cellBase + b];
                     if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
                     for (uint32_t j = 0; j < cnt; ++j) {
                         const uint32_t u = lo + j;
@@ -409,6 +419,7 @@ __global__ void __launch_bounds__(kBlock)
     // acceleration = force / density + gravity + fields (fluid_sph.cpp:260-261)
     float fa[3] = {0.0f, 0.0f, 0.0f};
     const float here[3] = {px, py, pz};
+    const BoundaryConsts& bnd = L.bounds[L.blockScene[xcd_block(blockIdx.x, gridDim.x)]];
     for (uint32_t f = 0; f < k.fieldCount; ++f) {
         float a[3];
         field_one(bnd.fields[f], here, a);
@@ -451,24 +462,20 @@ __global__ void __launch_bounds__(kBlock)
     velOut[t] = make_float4(vv[0], vv[1], vv[2], pressure);
 }
 
-inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
-
 }  // namespace
 
-hipError_t launch_density(const StepConsts& k, float4* pos, float4* vel, const uint32_t* bucketStart, const uint32_t* counts,
-                          uint4* nbrList, uint32_t* nbrCount, hipStream_t s) {
-    if (k.n == 0) return hipSuccess;
-    k_density<<<blocks_for(k.n), kBlock, 0, s>>>(k, pos, vel, bucketStart, counts, nbrList, nbrCount);
+hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, hipStream_t s) {
+    if (L.blocks == 0) return hipSuccess;
+    k_density<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount);
     return hipGetLastError();
 }
 
-hipError_t launch_force(const StepConsts& k, const BoundaryConsts& b, const float4* pos, const float4* vel,
-                        const uint32_t* bucketStart, const uint32_t* counts, const uint4* nbrList,
-                        const uint32_t* nbrCount, float4* posOut,
+hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
+                        const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s) {
-    if (k.n == 0) return hipSuccess;
-    k_force<<<blocks_for(k.n), kBlock, 0, s>>>(k, b, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut,
-                                               forceOut);
+    if (L.blocks == 0) return hipSuccess;
+    k_force<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut, forceOut);
     return hipGetLastError();
 }
 

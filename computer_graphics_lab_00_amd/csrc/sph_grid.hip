@@ -11,7 +11,10 @@
 // counting-sorted by their bucket's bin, and inside a bin ordered by (bucket, index).
 // Each bucket therefore stays contiguous and index-ascending, and neighbouring cells
 // sit next to each other in HBM, which is what the 27-bucket gathers read.
-//   1. clear:   counts[C] = 0, rep[C] = ~0, binCount[bins] = 0
+// All scenes of a batch go through each step in one launch (workgroups find their scene
+// in a table); the single solver is a batch of one.
+//   1. clear:   counts = 0, rep = ~0, bin counts = 0; each scene's filler bin = its
+//               unused capacity, so one global scan aligns every scene to its slots
 //   2. keys:    key = spatialHash(spatialCell(p, h)); counts[key] += 1; rep[key] min= bin
 //               (wave-level run aggregation: the state is already spatially sorted, so
 //               neighbouring lanes mostly share a bucket)
@@ -41,8 +44,8 @@ __device__ inline int32_t cell_coord(float x, float h) {
 
 __device__ inline uint32_t spread2(uint32_t v) { return (v & 1u) | ((v & 2u) << 2); }
 
-// Spatial bin of a cell: clamp into the bin grid, 4x4x4 tiles in x-fastest order,
-// Morton order of the 64 cells inside a tile.
+// Spatial bin of a cell: clamp into the bin grid, tiles of 2^tileLog2 bins in x-fastest
+// order, 64 bins per tile in Morton order (4x4x4 tiles) or x-fastest order (others).
 __device__ inline uint32_t bin_of(const BinGrid& g, int32_t cx, int32_t cy, int32_t cz) {
     const int32_t c[3] = {cx, cy, cz};
     uint32_t q[3];
@@ -53,22 +56,24 @@ __device__ inline uint32_t bin_of(const BinGrid& g, int32_t cx, int32_t cy, int3
         d >>= g.shift;
         q[a] = d >= (int64_t)g.dims[a] ? g.dims[a] - 1u : (uint32_t)d;
     }
-    const uint32_t tile = ((q[2] >> 2) * g.tiles[1] + (q[1] >> 2)) * g.tiles[0] + (q[0] >> 2);
-    const uint32_t m = spread2(q[0] & 3u) | (spread2(q[1] & 3u) << 1) | (spread2(q[2] & 3u) << 2);
+    const uint32_t lx = g.tileLog2[0], ly = g.tileLog2[1], lz = g.tileLog2[2];
+    const uint32_t tile = ((q[2] >> lz) * g.tiles[1] + (q[1] >> ly)) * g.tiles[0] + (q[0] >> lx);
+    const uint32_t ix = q[0] & ((1u << lx) - 1u), iy = q[1] & ((1u << ly) - 1u), iz = q[2] & ((1u << lz) - 1u);
+    uint32_t m;
+    if (lx == 2u && ly == 2u && lz == 2u) {
+        m = spread2(ix) | (spread2(iy) << 1) | (spread2(iz) << 2);
+    } else {
+        m = (iz << (lx + ly)) | (iy << lx) | ix;
+    }
     return tile * 64u + m;
 }
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
-// Bit mask of lanes strictly below / at or below this lane.
+// Bit mask of lanes at or below this lane.
 __device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ull : ((2ull << lane) - 1ull); }
 
-__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, uint32_t count) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t stride = gridDim.x * kBlock;
-    for (; i < count; i += stride) dst[i] = value;
-}
-
+// counts = 0 and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
 __global__ void __launch_bounds__(kBlock)
     k_grid_clear(uint32_t* __restrict__ counts, uint32_t* __restrict__ rep, uint32_t cells,
                  uint32_t* __restrict__ binCount, uint32_t bins) {
@@ -80,50 +85,59 @@ __global__ void __launch_bounds__(kBlock)
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < bins; i += stride) binCount[i] = 0u;
 }
 
+// Each scene's filler bin takes its unused capacity, so one scan over all bins puts every
+// scene's slots at its capacity-aligned offset.
+__global__ void __launch_bounds__(kBlock)
+    k_grid_fillers(Launch L, uint32_t* __restrict__ binCount) {
+    for (uint32_t s = threadIdx.x; s < L.numScenes; s += kBlock) {
+        const SceneDesc& d = L.scenes[s];
+        binCount[d.binOffset + d.binCapacity] = d.capacity - d.k.n;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ ids, uint32_t n) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) ids[i] = i;
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_keys(StepConsts k, BinGrid g, const float4* __restrict__ pos, uint32_t* __restrict__ keys,
-           uint32_t* __restrict__ counts, uint32_t* __restrict__ rep) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = s < k.n;
+    k_keys(Launch L, const float4* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ counts,
+           uint32_t* __restrict__ rep) {
+    const Where w = locate(L);
+    const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
     const uint32_t lane = lane_id();
     uint32_t key = 0xFFFFFFFFu, bin = 0xFFFFFFFFu;
-    if (active) {
-        const float4 p = pos[s];
+    if (w.valid) {
+        const float4 p = pos[w.slot];
         const int32_t cx = cell_coord(p.x, k.h), cy = cell_coord(p.y, k.h), cz = cell_coord(p.z, k.h);
-        key = hash_cell(cx, cy, cz, k.cellMask);
-        bin = bin_of(g, cx, cy, cz);
-        keys[s] = key;
+        key = w.d->cellOffset + hash_cell(cx, cy, cz, k.cellMask);
+        bin = w.d->binOffset + bin_of(w.d->g, cx, cy, cz);
+        keys[w.slot] = key;
     }
     // Runs of equal keys across consecutive lanes add their length with one atomic.
     const uint32_t prevKey = __shfl_up(key, 1, 64);
     const uint32_t prevBin = __shfl_up(bin, 1, 64);
-    const bool head = active && (lane == 0u || prevKey != key);
+    const bool head = w.valid && (lane == 0u || prevKey != key);
     const uint64_t heads = __ballot(head);
-    const uint64_t live = __ballot(active);
+    const uint64_t live = __ballot(w.valid);
     if (head) {
         const uint64_t above = heads & ~lanes_upto(lane);
         const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : (uint32_t)__popcll(live);
         atomicAdd(&counts[key], end - lane);
     }
-    if (active && (head || prevBin != bin)) atomicMin(&rep[key], bin);
+    if (w.valid && (head || prevBin != bin)) atomicMin(&rep[key], bin);
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_bin_slots(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rep,
+    k_bin_slots(Launch L, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rep,
                 uint32_t* __restrict__ binCount, uint32_t* __restrict__ binOf, uint32_t* __restrict__ slots) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = s < n;
+    const Where w = locate(L);
     const uint32_t lane = lane_id();
-    const uint32_t bin = active ? rep[keys[s]] : 0xFFFFFFFFu;
+    const uint32_t bin = w.valid ? rep[keys[w.slot]] : 0xFFFFFFFFu;
     const uint32_t prevBin = __shfl_up(bin, 1, 64);
-    const bool head = active && (lane == 0u || prevBin != bin);
+    const bool head = w.valid && (lane == 0u || prevBin != bin);
     const uint64_t heads = __ballot(head);
-    const uint64_t live = __ballot(active);
+    const uint64_t live = __ballot(w.valid);
     uint32_t base = 0;
     if (head) {
         const uint64_t above = heads & ~lanes_upto(lane);
@@ -134,9 +148,9 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t mine = heads & lanes_upto(lane);
     const uint32_t headLane = mine ? 63u - (uint32_t)__clzll((unsigned long long)mine) : lane;
     base = __shfl(base, headLane, 64);
-    if (active) {
-        binOf[s] = bin;
-        slots[s] = base + (lane - headLane);
+    if (w.valid) {
+        binOf[w.slot] = bin;
+        slots[w.slot] = base + (lane - headLane);
     }
 }
 
@@ -234,27 +248,29 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_scatter(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ binOf,
+    k_scatter(Launch L, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ binOf,
               const uint32_t* __restrict__ slots, const uint32_t* __restrict__ ids,
               const uint32_t* __restrict__ binStart, uint2* __restrict__ pairKI, uint32_t* __restrict__ pairSrc) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t dst = binStart[binOf[s]] + slots[s];
-    pairKI[dst] = make_uint2(keys[s], ids[s]);
-    pairSrc[dst] = s;
+    const Where w = locate(L);
+    if (!w.valid) return;
+    const uint32_t dst = binStart[binOf[w.slot]] + slots[w.slot];
+    pairKI[dst] = make_uint2(keys[w.slot], ids[w.slot]);
+    pairSrc[dst] = w.slot;
 }
 
-// Final slot = bin start + number of bin members before this one in (key, id) order.
+// Final slot = bin start + number of bin members before this one in (bucket, id) order.
+// Thread `slot` handles the pair scattered to that slot; a scene's n particles fill
+// exactly its first n slots.
 __global__ void __launch_bounds__(kBlock)
-    k_rank_reorder(uint32_t n, const uint2* __restrict__ pairKI, const uint32_t* __restrict__ pairSrc,
+    k_rank_reorder(Launch L, const uint2* __restrict__ pairKI, const uint32_t* __restrict__ pairSrc,
                    const uint32_t* __restrict__ binOf, const uint32_t* __restrict__ binStart,
                    const float4* __restrict__ pos, const float4* __restrict__ vel, float4* __restrict__ posSorted,
                    float4* __restrict__ velSorted, uint32_t* __restrict__ idsSorted,
                    uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    const uint2 me = pairKI[t];
-    const uint32_t src = pairSrc[t];
+    const Where w = locate(L);
+    if (!w.valid) return;
+    const uint2 me = pairKI[w.slot];
+    const uint32_t src = pairSrc[w.slot];
     const uint32_t bin = binOf[src];
     const uint32_t lo = binStart[bin], hi = binStart[bin + 1];
     uint32_t rank = 0;
@@ -280,10 +296,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_gather_keys(const uint32_t* __restrict__ keysSorted, const uint32_t* __restrict__ ids,
+    k_gather_keys(const uint32_t* __restrict__ keysSorted, const uint32_t* __restrict__ ids, uint32_t keyBase,
                   uint32_t* __restrict__ cellOf, uint32_t n) {
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t < n) cellOf[ids[t]] = keysSorted[t];
+    if (t < n) cellOf[ids[t]] = keysSorted[t] - keyBase;
 }
 
 struct EmitConsts {
@@ -328,35 +344,28 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
-hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t s) {
-    if (count == 0) return hipSuccess;
-    uint32_t blocks = blocks_for(count);
-    if (blocks > 8192u) blocks = 8192u;
-    k_fill_u32<<<blocks, kBlock, 0, s>>>(dst, value, count);
-    return hipGetLastError();
-}
-
 hipError_t launch_iota(uint32_t* ids, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     k_iota<<<blocks_for(n), kBlock, 0, s>>>(ids, n);
     return hipGetLastError();
 }
 
-hipError_t launch_grid_clear(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s) {
-    const uint32_t cells = k.cellMask + 1u;
-    uint32_t blocks = blocks_for(cells > g.bins ? cells : g.bins);
+hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t s) {
+    const uint32_t span = L.totalCells > L.totalBins ? L.totalCells : L.totalBins;
+    uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
-    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, cells, b.binStart, g.bins);
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins);
+    k_grid_fillers<<<1, kBlock, 0, s>>>(L, b.binStart);
     return hipGetLastError();
 }
 
-hipError_t launch_keys(const StepConsts& k, const BinGrid& g, const GridBuffers& b, hipStream_t s) {
-    k_keys<<<blocks_for(k.n), kBlock, 0, s>>>(k, g, b.pos, b.keys, b.counts, b.rep);
+hipError_t launch_keys(const Launch& L, const GridBuffers& b, hipStream_t s) {
+    k_keys<<<L.blocks, kBlock, 0, s>>>(L, b.pos, b.keys, b.counts, b.rep);
     return hipGetLastError();
 }
 
-hipError_t launch_bin_slots(const StepConsts& k, const GridBuffers& b, hipStream_t s) {
-    k_bin_slots<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, b.keys, b.rep, b.binStart, b.binOf, b.slots);
+hipError_t launch_bin_slots(const Launch& L, const GridBuffers& b, hipStream_t s) {
+    k_bin_slots<<<L.blocks, kBlock, 0, s>>>(L, b.keys, b.rep, b.binStart, b.binOf, b.slots);
     return hipGetLastError();
 }
 
@@ -368,16 +377,14 @@ hipError_t launch_scan(uint32_t* data, uint32_t count, uint32_t total, uint32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const StepConsts& k, const GridBuffers& b, hipStream_t s) {
-    k_scatter<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, b.keys, b.binOf, b.slots, b.ids, b.binStart, b.pairKI,
-                                                 b.pairSrc);
+hipError_t launch_scatter(const Launch& L, const GridBuffers& b, hipStream_t s) {
+    k_scatter<<<L.blocks, kBlock, 0, s>>>(L, b.keys, b.binOf, b.slots, b.ids, b.binStart, b.pairKI, b.pairSrc);
     return hipGetLastError();
 }
 
-hipError_t launch_rank_reorder(const StepConsts& k, const GridBuffers& b, hipStream_t s) {
-    k_rank_reorder<<<blocks_for(k.n), kBlock, 0, s>>>(k.n, b.pairKI, b.pairSrc, b.binOf, b.binStart, b.pos, b.vel,
-                                                      b.posSorted, b.velSorted, b.idsSorted, b.keysSorted,
-                                                      b.bucketStart);
+hipError_t launch_rank_reorder(const Launch& L, const GridBuffers& b, hipStream_t s) {
+    k_rank_reorder<<<L.blocks, kBlock, 0, s>>>(L, b.pairKI, b.pairSrc, b.binOf, b.binStart, b.pos, b.vel,
+                                              b.posSorted, b.velSorted, b.idsSorted, b.keysSorted, b.bucketStart);
     return hipGetLastError();
 }
 
@@ -387,10 +394,10 @@ hipError_t launch_unpermute(const float4* src, const uint32_t* ids, float4* dst,
     return hipGetLastError();
 }
 
-hipError_t launch_gather_keys(const uint32_t* keysSorted, const uint32_t* ids, uint32_t* cellOf, uint32_t n,
-                              hipStream_t s) {
+hipError_t launch_gather_keys(const uint32_t* keysSorted, const uint32_t* ids, uint32_t keyBase, uint32_t* cellOf,
+                              uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    k_gather_keys<<<blocks_for(n), kBlock, 0, s>>>(keysSorted, ids, cellOf, n);
+    k_gather_keys<<<blocks_for(n), kBlock, 0, s>>>(keysSorted, ids, keyBase, cellOf, n);
     return hipGetLastError();
 }
 

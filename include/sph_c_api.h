@@ -259,10 +259,47 @@ int sph_set_eos(SphSolver* solver, uint32_t eos);
 int sph_conservation(SphSolver* solver, const SphParams* params, double out[3]);
 
 /* Device time, in milliseconds, of each kernel class over the substeps run since the
- * last reset, measured with HIP events on the solver's stream when profiling is on.
+ * last reset, measured with HIP events on the solver's stream when profiling is on
+ * (profiling launches substeps one by one instead of replaying the step's hipGraph).
  * names/ms/launches hold up to `capacity` entries; returns the number written. */
 int sph_profile_enable(SphSolver* solver, int enable);
 int sph_profile_read(SphSolver* solver, const char** names, double* ms, uint32_t* launches, uint32_t capacity);
+
+/* ---- batches of independent scenes (no reference counterpart: the reference keeps one
+ * physics::FluidSolver per fluid, gfx::FluidSimulator::State, src/gfx/fluid.h:169-320) ----
+ *
+ * All scenes of a batch advance together: every kernel of a substep is one launch over
+ * all scenes, and a step (each scene with its own fluidSubsteps rule; a scene that needs
+ * fewer substeps than the batch's maximum passes through the extra ones unchanged) is
+ * captured once as a hipGraph and replayed. Each scene's results are bit-identical to
+ * running it alone on a solver with the same params and layout. */
+typedef struct SphBatch SphBatch;
+
+typedef struct SphSceneCapacity {
+    uint32_t maxParticles;
+    uint32_t maxCellCount;   /* power of two */
+} SphSceneCapacity;
+
+int sph_batch_create(const SphSceneCapacity* scenes, uint32_t sceneCount, int32_t device, uint32_t flags,
+                     SphBatch** out);
+int sph_batch_destroy(SphBatch* batch);
+uint32_t sph_batch_scene_count(const SphBatch* batch);
+/* The scene's FluidParams; required before emit / step. */
+int sph_batch_set_params(SphBatch* batch, uint32_t scene, const SphParams* params);
+/* Layout knobs (results do not depend on them): spatial-bin tile of 2^tileLog2 cells per
+ * axis (the three sum to 6) and whether the force pass uses the density pass's
+ * neighbour list. */
+int sph_batch_set_layout(SphBatch* batch, uint32_t scene, const uint32_t tileLog2[3], uint32_t neighbourList);
+int sph_batch_set_mode(SphBatch* batch, uint32_t scene, uint32_t modeFlags, uint32_t eos);
+int sph_batch_emit(SphBatch* batch, uint32_t scene, uint32_t count);
+int sph_batch_load_state(SphBatch* batch, uint32_t scene, const float* pos4, const float* vel4, uint32_t count);
+/* FluidSolver::step for every scene; dt <= 0 is a no-op. Enqueued, returns early. */
+int sph_batch_step(SphBatch* batch, float deltaSeconds);
+int sph_batch_synchronize(SphBatch* batch);
+uint32_t sph_batch_particle_count(const SphBatch* batch, uint32_t scene);
+int sph_batch_read_particles(SphBatch* batch, uint32_t scene, float* pos4, float* vel4);
+int sph_batch_profile_enable(SphBatch* batch, int enable);
+int sph_batch_profile_read(SphBatch* batch, const char** names, double* ms, uint32_t* launches, uint32_t capacity);
 
 #ifdef __cplusplus
 }

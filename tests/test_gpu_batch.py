@@ -1,0 +1,108 @@
+"""GPU: batched scenes equal the same scenes run alone, bit for bit; the hipGraph replay
+equals eager launches; the autotuner's choice comes back from its cache."""
+import sys
+
+import numpy as np
+import pytest
+
+from computer_graphics_lab_00_amd import batch, capi, fluid
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 600.0
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def small_specs():
+    specs = batch.batch64(11, scenes=6, n_min=2000, n_max=20000)
+    # different stiffness -> different substep counts inside one batch
+    specs[1].stiffness = 200.0
+    specs[2].stiffness = 20000.0
+    specs[3].tile = (3, 3, 0)
+    specs[4].neighbour_list = False
+    specs[5].cell_count = 4 * batch.bit_ceil(2 * specs[5].count)
+    return specs
+
+
+def run_alone(spec, steps):
+    p = spec.params()
+    pos, vel = spec.initial_state(p)
+    flags = 0 if spec.neighbour_list else capi.CFG_BUCKET_WALK
+    with fluid.FluidSolver(spec.count, max(p.cellCount, 1024), flags=flags) as s:
+        s.load_state(pos, vel)
+        for _ in range(steps):
+            s.step(p, DT)
+        return s.state()
+
+
+def test_batch_equals_scenes_run_alone():
+    specs = small_specs()
+    subs = [fluid.fluid_substeps(s.params(), DT) for s in specs]
+    say(f"case: batch of {len(specs)} scenes, substeps {subs}")
+    assert len(set(subs)) > 1, "the batch should mix substep counts"
+    with batch.SceneBatch(specs) as b:
+        b.load_initial()
+        for _ in range(4):
+            b.step(DT)
+        got = [b.state(i) for i in range(len(specs))]
+    for i, s in enumerate(specs):
+        alone = run_alone(s, 4)
+        assert np.array_equal(bits(got[i][0]), bits(alone[0])), f"scene {i} positions"
+        assert np.array_equal(bits(got[i][1]), bits(alone[1])), f"scene {i} velocities"
+
+
+def test_graph_replays_equal_eager_launches():
+    """The step's hipGraph replayed several times, with reads and synchronisations in
+    between, against the same batch launched eagerly."""
+    specs = small_specs()[:4]
+    say("case: graph vs eager")
+    with batch.SceneBatch(specs) as g, batch.SceneBatch(specs, flags=capi.CFG_NO_GRAPH) as e:
+        g.load_initial()
+        e.load_initial()
+        for k in range(5):
+            g.step(DT)
+            e.step(DT)
+            for i in range(len(specs)):
+                a, b = g.state(i), e.state(i)
+                assert np.array_equal(bits(a[0]), bits(b[0])), f"replay {k} scene {i}"
+                assert np.array_equal(bits(a[1]), bits(b[1])), f"replay {k} scene {i}"
+
+
+def test_solver_graph_equals_eager():
+    say("case: solver graph vs eager")
+    w = fluid.s16k()
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    outs = []
+    for flags in (0, capi.CFG_NO_GRAPH):
+        with fluid.FluidSolver(w.count, 1 << 16, flags=flags) as s:
+            s.load_state(pos, vel)
+            for _ in range(7):          # 7 steps x 5 substeps: odd and even id parities
+                s.step(p, DT)
+                s.particles()
+            outs.append(s.state())
+    assert np.array_equal(bits(outs[0][0]), bits(outs[1][0]))
+    assert np.array_equal(bits(outs[0][1]), bits(outs[1][1]))
+
+
+def test_autotune_then_cache(tmp_path):
+    specs = batch.batch64(13, scenes=2, n_min=2000, n_max=4000)
+    say("case: autotune 2 scenes")
+    cache = tmp_path / "tune.json"
+    first = batch.autotune(specs, cache, steps=2, warmup=1)
+    assert cache.exists()
+    again = batch.autotune(specs, cache)
+    assert [(a.cell_count, a.tile, a.neighbour_list) for a in first] == \
+           [(b.cell_count, b.tile, b.neighbour_list) for b in again]
+    # the tuned scenes still run and stay finite
+    with batch.SceneBatch(again) as b:
+        b.load_initial()
+        b.step(DT)
+        assert np.isfinite(b.state(0)[0]).all()
