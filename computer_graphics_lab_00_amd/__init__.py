@@ -7,4 +7,4 @@ missing.
 """
 from .capi import SphCollider, SphError, SphForceField, SphParams, SphSettings  # noqa: F401
 from .fluid import (FluidSolver, Workload, cell_count_for, derive_fluid_params, emit_lattice,  # noqa: F401
-                    fluid_substeps, lattice_jitter, m1, s16k, spatial_hash, translation4)
+                    fluid_substeps, lattice_jitter, m1, r4m, s16k, spatial_hash, translation4)

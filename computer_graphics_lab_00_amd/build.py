@@ -27,6 +27,9 @@ SOURCES = [
     "sph_forces.hip",
     "sph_engine.hip",
     "sph_api.hip",
+    "sph_mlp.hip",
+    "sph_surrogate_kernels.hip",
+    "sph_surrogate.hip",
 ]
 
 COMMON_FLAGS = [

@@ -134,6 +134,16 @@ class SphSceneCapacity(C.Structure):
     _fields_ = [("maxParticles", C.c_uint32), ("maxCellCount", C.c_uint32)]
 
 
+class SphRolloutStats(C.Structure):
+    _fields_ = [
+        ("macroSteps", C.c_uint64),
+        ("fallbacks", C.c_uint64),
+        ("plainSubsteps", C.c_uint64),
+        ("maxDeviation", C.c_float),
+        ("lastDeviation", C.c_float),
+    ]
+
+
 class SphError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -145,6 +155,7 @@ _lib = None
 _FP = C.POINTER(C.c_float)
 _UP = C.POINTER(C.c_uint32)
 _DP = C.POINTER(C.c_double)
+_HP = C.POINTER(C.c_uint16)
 
 
 def _declare(lib):
@@ -195,6 +206,30 @@ def _declare(lib):
         "sph_batch_profile_read": (C.c_int, [P, C.POINTER(C.c_char_p), _DP, _UP, C.c_uint32]),
         "sph_debug_step_constants": (C.c_int, [C.POINTER(SphParams), C.c_float, _FP]),
         "sph_debug_copy": (C.c_int, [C.c_uint64]),
+        "sph_debug_gemm": (C.c_int, [_HP, _HP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _FP]),
+        "sph_net_param_count": (C.c_uint32, []),
+        "sph_net_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(P)]),
+        "sph_net_destroy": (C.c_int, [P]),
+        "sph_net_get_params": (C.c_int, [P, _FP]),
+        "sph_net_set_params": (C.c_int, [P, _FP]),
+        "sph_net_save": (C.c_int, [P, C.c_char_p]),
+        "sph_net_load": (C.c_int, [P, C.c_char_p]),
+        "sph_net_forward": (C.c_int, [P, _HP, C.c_uint32, _FP]),
+        "sph_net_gradients": (C.c_int, [P, _HP, _FP, C.c_uint32, _FP, _FP]),
+        "sph_net_train_batch": (C.c_int, [P, _HP, _FP, C.c_uint32, C.c_float, _FP]),
+        "sph_dataset_create": (C.c_int, [C.c_int32, C.c_uint32, C.POINTER(P)]),
+        "sph_dataset_destroy": (C.c_int, [P]),
+        "sph_dataset_size": (C.c_uint32, [P]),
+        "sph_dataset_record": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_uint64]),
+        "sph_dataset_read": (C.c_int, [P, C.c_uint32, C.c_uint32, _HP, _FP, _FP]),
+        "sph_net_train": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                    _FP]),
+        "sph_net_evaluate": (C.c_int, [P, P, C.c_uint32, C.c_uint32, _DP]),
+        "sph_features": (C.c_int, [P, C.POINTER(SphParams), C.c_float, _HP]),
+        "sph_rollout": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_float,
+                                  C.POINTER(SphRolloutStats)]),
+        "sph_net_inference_time": (C.c_int, [P, _DP, C.POINTER(C.c_uint64)]),
+        "sph_density_deviation": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, _DP]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -13,14 +13,6 @@
 using sphhost::Engine;
 using sphhost::fail;
 
-struct SphSolver {
-    Engine e;
-};
-
-struct SphBatch {
-    Engine e;
-};
-
 namespace {
 
 int conservation_of(Engine& e, uint32_t scene, const SphParams* p, double out[3]) {

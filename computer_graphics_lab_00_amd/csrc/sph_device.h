@@ -168,7 +168,8 @@ hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hip
 
 // launchers (sph_forces.hip)
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, hipStream_t s);
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, uint32_t* knn,
+                          const uint32_t* ids, hipStream_t s);
 hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
                         const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s);

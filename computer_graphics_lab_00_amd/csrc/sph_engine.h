@@ -69,6 +69,30 @@ public:
 
     int set_device() const;
 
+    // ---- hooks of the learned surrogate ----
+    // Grid build + density pass of one substep of length dt on the current state (with
+    // the 8 nearest neighbours per particle into knn[8 x total] when knn is set). The
+    // sorted state is left in the back buffers; nothing is committed.
+    int prepare(float dt, uint32_t* knn);
+    float4* sorted_pos() { return pos[1 - pc]; }
+    float4* sorted_vel() { return vel[1 - pc]; }
+    uint32_t* sorted_ids() { return ids[1 - ic]; }
+    float4* state_pos() { return pos[pc]; }
+    float4* state_vel() { return vel[pc]; }
+    uint32_t* state_ids() { return ids[ic]; }
+    float4* force_buffer() { return force; }
+    // After a kernel wrote the state buffers from the prepared (sorted) ones.
+    void commit_sorted() {
+        ic = 1 - ic;
+        hasGrid = true;
+    }
+    sphdev::Launch plan_launch() const { return launch_for(0); }
+    uint32_t total_slots() const { return totalN; }
+    int ensure_force_buffer();
+    int backup_state();
+    int restore_state();
+    uint32_t planned_blocks() const { return planBlocks; }
+
     std::vector<SceneHost> sc;
     int device = 0;
     uint32_t flags = 0;
@@ -77,7 +101,7 @@ public:
 
 private:
     int ensure_plan(float dt, bool singleSubstep, uint32_t* maxSub);
-    int run_substep(uint32_t j);
+    int run_substep(uint32_t j, bool prepareOnly = false, uint32_t* knn = nullptr);
     int enqueue_step(uint32_t maxSub);
     void invalidate();
     sphdev::Launch launch_for(uint32_t j) const;
@@ -95,6 +119,8 @@ private:
     uint32_t* nbrCount = nullptr;
     float4 *prev = nullptr, *force = nullptr, *staging = nullptr;
     uint32_t* stagingU = nullptr;
+    float4 *backupPos = nullptr, *backupVel = nullptr;
+    uint32_t* backupIds = nullptr;
     // plan: per-substep scene tables, boundaries, block -> scene
     sphdev::SceneDesc* dTables = nullptr;
     sphdev::BoundaryConsts* dBounds = nullptr;
@@ -122,6 +148,15 @@ sphdev::BinGrid make_bins(const SceneHost& s, const SphParams& p);
 int hip_fail(hipError_t e, const char* what);
 
 }  // namespace sphhost
+
+// The C-ABI handles are engines: a solver has one scene, a batch many.
+struct SphSolver {
+    sphhost::Engine e;
+};
+
+struct SphBatch {
+    sphhost::Engine e;
+};
 
 #define SPH_HIP(call)                                                     \
     do {                                                                  \

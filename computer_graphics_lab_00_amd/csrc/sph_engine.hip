@@ -199,6 +199,9 @@ void Engine::free_all() {
     hipFree(force);
     hipFree(staging);
     hipFree(stagingU);
+    hipFree(backupPos);
+    hipFree(backupVel);
+    hipFree(backupIds);
     hipFree(dTables);
     hipFree(dBounds);
     hipFree(dBlockScene);
@@ -466,7 +469,7 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     return SPH_OK;
 }
 
-int Engine::run_substep(uint32_t j) {
+int Engine::run_substep(uint32_t j, bool prepareOnly, uint32_t* knn) {
     const sphdev::Launch L = launch_for(j);
     sphdev::GridBuffers g;
     g.pos = pos[pc];
@@ -512,8 +515,10 @@ int Engine::run_substep(uint32_t j) {
     }
     {
         Timer t(this, KC_DENSITY);
-        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, stream));
+        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, knn,
+                                       g.idsSorted, stream));
     }
+    if (prepareOnly) return SPH_OK;
     {
         Timer t(this, KC_FORCE);
         SPH_HIP(sphdev::launch_force(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, pos[pc],
@@ -559,6 +564,44 @@ int Engine::step(float dt, bool singleSubstep) {
         ic = (maxSub & 1u) ? 1 - parity : parity;
     }
     hasGrid = true;
+    return SPH_OK;
+}
+
+int Engine::prepare(float dt, uint32_t* knn) {
+    if (int st = set_device()) return st;
+    uint32_t maxSub = 0;
+    if (int st = ensure_plan(dt, true, &maxSub)) return st;
+    if (maxSub == 0) return SPH_OK;
+    return run_substep(0, true, knn);
+}
+
+int Engine::ensure_force_buffer() {
+    if (force) return SPH_OK;
+    if (int st = set_device()) return st;
+    SPH_HIP(hipMalloc(&force, (size_t)totalN * sizeof(float4)));
+    invalidate();   // captured graphs wrote no forces
+    return SPH_OK;
+}
+
+int Engine::backup_state() {
+    if (int st = set_device()) return st;
+    if (!backupPos) {
+        SPH_HIP(hipMalloc(&backupPos, (size_t)totalN * sizeof(float4)));
+        SPH_HIP(hipMalloc(&backupVel, (size_t)totalN * sizeof(float4)));
+        SPH_HIP(hipMalloc(&backupIds, (size_t)totalN * sizeof(uint32_t)));
+    }
+    SPH_HIP(hipMemcpyAsync(backupPos, pos[pc], (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    SPH_HIP(hipMemcpyAsync(backupVel, vel[pc], (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    SPH_HIP(hipMemcpyAsync(backupIds, ids[ic], (size_t)totalN * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    return SPH_OK;
+}
+
+int Engine::restore_state() {
+    if (!backupPos) return fail(SPH_ERR_STATE, "no backup to restore");
+    if (int st = set_device()) return st;
+    SPH_HIP(hipMemcpyAsync(pos[pc], backupPos, (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    SPH_HIP(hipMemcpyAsync(vel[pc], backupVel, (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    SPH_HIP(hipMemcpyAsync(ids[ic], backupIds, (size_t)totalN * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
     return SPH_OK;
 }
 

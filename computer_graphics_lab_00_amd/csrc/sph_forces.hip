@@ -68,9 +68,20 @@ __device__ inline float eos_pressure(const StepConsts& k, float density) {
 // at groups[g * n + t], so the force pass reads a whole wave's group g in one coalesced
 // access. A particle with more than `listCap` entries stores count = listCap + 1 and the
 // force pass scans its buckets instead.
+// Lexicographic (r2, particle index) order of the nearest-neighbour selection: ties
+// are broken by index, so the choice does not depend on the memory layout.
+__device__ __forceinline__ bool nearer(float r2, uint32_t id, float r2b, uint32_t idb) {
+    return r2 < r2b || (r2 == r2b && id < idb);
+}
+
+// With KNN the pass also keeps, per particle, the 8 nearest in-range candidates other
+// than itself by (r2, index) — a duplicate visit of a bucket cannot enter twice — and
+// writes their slots to knn[k * total + t] (0xFFFFFFFF where fewer than 8 exist).
+template <bool KNN>
 __global__ void __launch_bounds__(kBlock)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
-              const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount) {
+              const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
+              uint32_t* __restrict__ knn, const uint32_t* __restrict__ knnIds) {
     const Where w = locate(L);
     if (!w.valid || !w.d->active) return;
     const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
@@ -87,6 +98,14 @@ __global__ void __launch_bounds__(kBlock)
     float density = 0.0f;
     uint32_t listed = 0;
     uint4 group = make_uint4(0u, 0u, 0u, 0u);
+    float kd[8];
+    uint32_t ks[8], ki[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        kd[q] = INFINITY;
+        ks[q] = 0xFFFFFFFFu;
+        ki[q] = 0xFFFFFFFFu;
+    }
     for (int32_t dz = -1; dz <= 1; ++dz) {
         // The nine buckets of one z layer: all descriptors are fetched before any is used.
         uint32_t bk[9], cn[9], st[9];
@@ -131,6 +150,28 @@ __global__ void __launch_bounds__(kBlock)
                             if (lane4 == 3u) store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
                         }
                         ++listed;
+                        if (KNN && r2 <= kd[7]) {
+                            const uint32_t u = lo + j + e;
+                            bool dup = u == t;
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) dup |= ks[q] == u;
+                            const uint32_t id = dup ? 0xFFFFFFFFu : knnIds[u];
+                            if (!dup && nearer(r2, id, kd[7], ki[7])) {
+#pragma unroll
+                                for (int q = 7; q > 0; --q) {
+                                    const bool up = nearer(r2, id, kd[q - 1], ki[q - 1]);
+                                    const bool here = nearer(r2, id, kd[q], ki[q]);
+                                    kd[q] = up ? kd[q - 1] : (here ? r2 : kd[q]);
+                                    ks[q] = up ? ks[q - 1] : (here ? u : ks[q]);
+                                    ki[q] = up ? ki[q - 1] : (here ? id : ki[q]);
+                                }
+                                if (nearer(r2, id, kd[0], ki[0])) {
+                                    kd[0] = r2;
+                                    ks[0] = u;
+                                    ki[0] = id;
+                                }
+                            }
+                        }
                     }
                 }
             }
@@ -143,6 +184,10 @@ __global__ void __launch_bounds__(kBlock)
         if (listed <= cap && (listed & 3u))
             store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
         nbrCount[t] = listed <= cap ? listed : cap + 1u;
+    }
+    if (KNN) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) knn[(size_t)q * L.total + t] = ks[q];
     }
 }
 
@@ -465,9 +510,15 @@ __global__ void __launch_bounds__(kBlock)
 }  // namespace
 
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, hipStream_t s) {
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, uint32_t* knn,
+                          const uint32_t* ids, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_density<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount);
+    if (knn) {
+        k_density<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, knn, ids);
+    } else {
+        k_density<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, nullptr,
+                                                     nullptr);
+    }
     return hipGetLastError();
 }
 

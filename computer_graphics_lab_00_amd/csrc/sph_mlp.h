@@ -1,0 +1,58 @@
+// sph_mlp.h — the surrogate network's kernels (BF16 operands, FP32 accumulate) and the
+// BF16 rounding shared by host and device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+namespace sphmlp {
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet), on host and device alike.
+__host__ __device__ inline uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf16_bits_to_f32(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// C[M x N] = A[M x K] * B[K x N], operands BF16 read through element strides.
+// M and N multiples of 32, K and kPerSplit multiples of 16. With `partial` set, K is
+// split into `splits` chunks of kPerSplit and chunk c writes partial[c][M][N]; otherwise
+// the epilogue adds bias[n], applies the backward ReLU mask and/or ReLU, and writes
+// FP32 and/or BF16 rows of stride ldOut.
+struct GemmArgs {
+    const uint16_t* A;
+    const uint16_t* B;
+    uint32_t M, N, K;
+    uint32_t sAm, sAk, sBk, sBn;
+    uint32_t splits, kPerSplit;
+    float* partial;
+    const float* bias;
+    const uint16_t* reluMask;   // forward output (BF16) whose positive entries pass the gradient
+    int relu;
+    float* outF32;
+    uint16_t* outBf16;
+    uint32_t ldOut;
+};
+
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
+hipError_t launch_reduce_partials(const float* partial, uint32_t splits, uint32_t count, float* out, hipStream_t s);
+hipError_t launch_mse_grad(const float* y, const float* target, uint32_t rows, uint32_t rowsPadded, uint32_t ld,
+                           uint32_t outs, float scale, uint16_t* dy, float* rowErr, hipStream_t s);
+// Column sums over `rows` rows; `partial` holds ceil(rows / 256) * cols floats.
+hipError_t launch_column_sum_bf16(const uint16_t* m, uint32_t rows, uint32_t cols, float* partial, float* out,
+                                  hipStream_t s);
+hipError_t launch_adam(float* w, float* m1, float* m2, const float* grad, uint32_t count, float lr, float b1,
+                       float b2, float eps, float c1, float c2, uint16_t* wBf16, const float* frozen, hipStream_t s);
+hipError_t launch_to_bf16(const float* src, uint32_t count, uint16_t* dst, hipStream_t s);
+
+}  // namespace sphmlp

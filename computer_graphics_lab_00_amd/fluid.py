@@ -214,6 +214,7 @@ class Workload:
     cell_count: int = 1024
     jitter_seed: int = 12345
     jitter: float | None = None   # amplitude in metres; default 0.05 * spacing
+    corner: tuple[float, float, float] | None = None   # lowest lattice point; default box_min + r
 
     @property
     def count(self) -> int:
@@ -229,7 +230,8 @@ class Workload:
             particleCount=self.count, particleRadius=r, smoothingRadius=self.smoothing, restDensity=1000.0,
             stiffness=self.stiffness, viscosity=self.viscosity, containerMin=self.box_min,
             containerMax=self.box_max, gravity=[0.0, -9.81, 0.0])
-        center = [self.box_min[a] + r + (self.lattice[a] - 1) * self.spacing * 0.5 for a in range(3)]
+        low = self.corner if self.corner is not None else [self.box_min[a] + r for a in range(3)]
+        center = [low[a] + (self.lattice[a] - 1) * self.spacing * 0.5 for a in range(3)]
         return derive_fluid_params(settings, translation4(*center), self.count, self.cell_count)
 
     def initial_state(self, params: SphParams | None = None) -> tuple[np.ndarray, np.ndarray]:
@@ -273,4 +275,12 @@ def s16k(cell_count: int = 32768) -> Workload:
 def m1(cell_count: int = 1 << 21) -> Workload:
     """1,048,576 particles, 128x128x64 at 0.0078 m, h = 0.0156 m, in a 2 x 1 x 1 m box."""
     return Workload("dam_break_1m", (128, 128, 64), 0.0078, (0.0, 0.0, 0.0), (2.0, 1.0, 1.0), 0.0156,
+                    cell_count=cell_count)
+
+
+def r4m(cell_count: int = 1 << 23) -> Workload:
+    """4,194,304 particles, 256x128x128 at 0.0049 m, h = 0.0098 m, in a 2.5 x 0.64 x 1.25 m
+    box: the 1M dam break's proportions (half the length, the full height, half the
+    depth) at a finer resolution."""
+    return Workload("dam_break_4m", (256, 128, 128), 0.0049, (0.0, 0.0, 0.0), (2.5, 0.64, 1.25), 0.0098,
                     cell_count=cell_count)

@@ -1,0 +1,312 @@
+"""The learned surrogate of the SPH step, host side.
+
+No counterpart in the reference: this sits on top of ``physics::FluidSolver::step``
+(src/physics/fluid_sph.cpp:151-303) and keeps that path as its baseline and judge.
+
+* ``SurrogateNet``: the MLP 64 -> 128 -> 128 -> 128 -> 3 (ReLU) that maps a particle's
+  neighbourhood features to its SPH acceleration; trained and evaluated on the GPU by
+  the library's BF16 MFMA kernels (FP32 accumulate, FP32 master weights, Adam).
+* ``TrainingSet``: device-resident (features, target) rows recorded from plain-SPH
+  trajectories; the target is the mean SPH acceleration over a 4-substep macro-step.
+* ``training_scene`` / ``generate``: the seeded scenes the trajectories come from —
+  random lattices with velocities drawn from N(0, 0.5 m/s).
+* ``rollout``: ``FluidSolver.step`` with every 4 substeps replaced by one network
+  evaluation, checked by the next density pass and redone by the plain path when the
+  mean density excess is above the threshold (1 % by default).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import sys
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import capi
+from .capi import SphParams, SphRolloutStats, check
+from .fluid import FluidSolver, Workload, cell_count_for, fluid_substeps
+
+FEATURES = 64
+OUTPUTS = 3
+MACRO = 4
+DEFAULT_THRESHOLD = 0.01
+_FP = C.POINTER(C.c_float)
+_HP = C.POINTER(C.c_uint16)
+_DP = C.POINTER(C.c_double)
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _hp(a):
+    return None if a is None else a.ctypes.data_as(_HP)
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_FP)
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """The first `count` outputs of splitmix64 (Steele, Lea & Flood 2014) from `seed`."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + _GOLDEN * np.arange(1, count + 1, dtype=np.uint64)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def gaussian(seed: int, count: int, sigma: float = 1.0) -> np.ndarray:
+    """`count` N(0, sigma^2) samples: Box-Muller (1958) on pairs of splitmix64 uniforms."""
+    pairs = (count + 1) // 2
+    z = splitmix64(seed, 2 * pairs)
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log1p(-u1))   # 1 - u1 is in (0, 1]
+    g = np.empty(2 * pairs)
+    g[0::2] = r * np.cos(2.0 * math.pi * u2)
+    g[1::2] = r * np.sin(2.0 * math.pi * u2)
+    return (g[:count] * sigma).astype(np.float32)
+
+
+class SurrogateNet:
+    """The network on one GPU; parameters W[out][in] then b[out] per layer."""
+
+    def __init__(self, seed: int = 1, device: int = 0):
+        self._lib = capi.load()
+        h = C.c_void_p()
+        check(self._lib.sph_net_create(device, seed, C.byref(h)))
+        self._h = h
+        self.param_count = int(self._lib.sph_net_param_count())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sph_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.param_count, np.float32)
+        check(self._lib.sph_net_get_params(self._h, _fp(out)))
+        return out
+
+    @params.setter
+    def params(self, values) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if v.shape != (self.param_count,):
+            raise ValueError(f"expected {self.param_count} parameters")
+        check(self._lib.sph_net_set_params(self._h, _fp(v)))
+
+    def save(self, path) -> None:
+        check(self._lib.sph_net_save(self._h, str(path).encode()))
+
+    def load(self, path) -> None:
+        check(self._lib.sph_net_load(self._h, str(path).encode()))
+
+    def forward(self, x_bits: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x_bits, dtype=np.uint16)
+        y = np.empty((x.shape[0], OUTPUTS), np.float32)
+        check(self._lib.sph_net_forward(self._h, _hp(x), x.shape[0], _fp(y)))
+        return y
+
+    def gradients(self, x_bits: np.ndarray, targets: np.ndarray) -> tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(x_bits, dtype=np.uint16)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(self.param_count, np.float32)
+        loss = C.c_float()
+        check(self._lib.sph_net_gradients(self._h, _hp(x), _fp(t), x.shape[0], _fp(g), C.byref(loss)))
+        return g, float(loss.value)
+
+    def train_batch(self, x_bits: np.ndarray, targets: np.ndarray, lr: float = 1e-3) -> float:
+        x = np.ascontiguousarray(x_bits, dtype=np.uint16)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = C.c_float()
+        check(self._lib.sph_net_train_batch(self._h, _hp(x), _fp(t), x.shape[0], lr, C.byref(loss)))
+        return float(loss.value)
+
+    def train(self, data: "TrainingSet", begin: int, end: int, batch: int, steps: int, lr: float = 1e-3,
+              seed: int = 7, losses: bool = False) -> np.ndarray | None:
+        out = np.empty(steps, np.float32) if losses else None
+        check(self._lib.sph_net_train(self._h, data._h, begin, end, batch, steps, lr, seed, _fp(out)))
+        return out
+
+    def evaluate(self, data: "TrainingSet", begin: int, end: int) -> dict[str, float]:
+        out = (C.c_double * 3)()
+        check(self._lib.sph_net_evaluate(self._h, data._h, begin, end, out))
+        return {"network": out[0], "gravity_only": out[1], "hold_first_substep": out[2]}
+
+    def inference_time(self) -> tuple[float, int]:
+        ms = C.c_double()
+        rows = C.c_uint64()
+        check(self._lib.sph_net_inference_time(self._h, C.byref(ms), C.byref(rows)))
+        return ms.value, rows.value
+
+
+class TrainingSet:
+    """(feature row, target) pairs resident on the GPU."""
+
+    def __init__(self, capacity: int, device: int = 0):
+        self._lib = capi.load()
+        h = C.c_void_p()
+        check(self._lib.sph_dataset_create(device, capacity, C.byref(h)))
+        self._h = h
+        self.capacity = capacity
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sph_dataset_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return int(self._lib.sph_dataset_size(self._h))
+
+    def record(self, solver: FluidSolver, params: SphParams, dt_sub: float, samples: int, seed: int) -> None:
+        """One macro-step of the plain path (advances the solver by 4 substeps)."""
+        check(self._lib.sph_dataset_record(self._h, solver._h, C.byref(params), dt_sub, samples, seed))
+
+    def read(self, begin: int = 0, count: int | None = None):
+        count = len(self) - begin if count is None else count
+        x = np.empty((count, FEATURES), np.uint16)
+        t = np.empty((count, OUTPUTS), np.float32)
+        t0 = np.empty((count, OUTPUTS), np.float32)
+        check(self._lib.sph_dataset_read(self._h, begin, count, _hp(x), _fp(t), _fp(t0)))
+        return x, t, t0
+
+
+def features(solver: FluidSolver, params: SphParams, dt_sub: float) -> np.ndarray:
+    """Feature rows (BF16 bits) of every particle of the current state, particle order."""
+    n = solver.particle_count()
+    x = np.empty((n, FEATURES), np.uint16)
+    check(capi.load().sph_features(solver._h, C.byref(params), dt_sub, _hp(x)))
+    return x
+
+
+def density_deviation(solver: FluidSolver, net: SurrogateNet, params: SphParams, dt_sub: float) -> tuple[float, float]:
+    """(mean max(rho / rho0 - 1, 0), mean |rho / rho0 - 1|) of the current state."""
+    out = (C.c_double * 2)()
+    check(capi.load().sph_density_deviation(solver._h, net._h, C.byref(params), dt_sub, out))
+    return out[0], out[1]
+
+
+def rollout(solver: FluidSolver, net: SurrogateNet, params: SphParams, dt: float, steps: int = 1,
+            threshold: float = DEFAULT_THRESHOLD, stats: SphRolloutStats | None = None) -> SphRolloutStats:
+    """`steps` calls of FluidSolver.step(dt) with the surrogate; returns the counters
+    (accumulated into `stats` when given)."""
+    st = stats if stats is not None else SphRolloutStats()
+    check(capi.load().sph_rollout(solver._h, net._h, C.byref(params), dt, steps, threshold, C.byref(st)))
+    return st
+
+
+# ---- training scenes ----
+
+LATTICES = ((40, 40, 40), (64, 32, 32), (32, 64, 32), (32, 32, 64))
+RADII = (0.0098, 0.0156)   # kernel radii of the 4M and 1M dam breaks; spacing = h / 2
+
+
+@dataclass
+class TrainingScene:
+    workload: Workload
+    velocity_seed: int
+    warmup_steps: int
+
+    def initial_state(self):
+        p = self.workload.params()
+        pos, _ = self.workload.initial_state(p)
+        vel = np.zeros_like(pos)
+        vel[:, :3] = gaussian(self.velocity_seed, pos.shape[0] * 3, 0.5).reshape(-1, 3)
+        return p, pos, vel
+
+
+def training_scene(index: int, seed: int = 2024) -> TrainingScene:
+    """Scene `index` of the seeded training family: one of four 64k lattices at one of the
+    two resolutions, placed at a random spot of a box 1.2-2x its size (resting on the
+    floor half of the time), with velocities from N(0, 0.5 m/s)."""
+    u = (splitmix64(seed * 1000003 + index, 8) >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+    lattice = LATTICES[index % len(LATTICES)]
+    h = RADII[(index // len(LATTICES)) % len(RADII)]
+    spacing = h * 0.5
+    extent = [n * spacing for n in lattice]
+    box = [extent[a] * (1.2 + 0.8 * u[a]) for a in range(3)]
+    r = spacing * 0.5
+    corner = []
+    for a in range(3):
+        slack = box[a] - extent[a] - 2 * r
+        lo = r + slack * u[3 + a]
+        if a == 1 and u[6] < 0.5:
+            lo = r
+        corner.append(lo)
+    count = lattice[0] * lattice[1] * lattice[2]
+    wl = Workload(f"train_{index}", lattice, spacing, (0.0, 0.0, 0.0), tuple(box), h,
+                  cell_count=max(1024, 1 << int(math.ceil(math.log2(2 * count)))),
+                  jitter_seed=(seed + index) & 0xFFFFFFFF, corner=tuple(corner))
+    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * 12))
+
+
+def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4096, dt: float = 1.0 / 600.0,
+             gap_steps: int = 1, seed: int = 2024, first: int = 0, log=None) -> list[tuple[int, int]]:
+    """Records `records` macro-steps of each of scenes first..first+scenes-1 into `data`;
+    returns each scene's row range. Between records the scene runs `gap_steps` plain
+    steps so the rows of one scene are not all alike."""
+    ranges = []
+    solver = None
+    cap = 0
+    for i in range(first, first + scenes):
+        sc = training_scene(i, seed)
+        p, pos, vel = sc.initial_state()
+        n = pos.shape[0]
+        if solver is None or n > cap or sc.workload.cell_count > solver.max_cell_count:
+            if solver is not None:
+                solver.close()
+            cap = max(n, 65536)
+            solver = FluidSolver(cap, max(sc.workload.cell_count, 1 << 18), keep_forces=True)
+        solver.load_state(pos, vel)
+        sub = fluid_substeps(p, dt)
+        dt_sub = min(dt, 1.0 / 30.0) / sub
+        for _ in range(sc.warmup_steps):
+            solver.step(p, dt)
+        begin = len(data)
+        for r in range(records):
+            data.record(solver, p, dt_sub, min(samples, n), seed=(seed << 20) + i * 1000 + r)
+            for _ in range(gap_steps):
+                solver.step(p, dt)
+        ranges.append((begin, len(data)))
+        if log is not None and (i - first) % 20 == 19:
+            print(f"[surrogate] recorded {i - first + 1}/{scenes} scenes, {len(data)} rows", file=log, flush=True)
+    if solver is not None:
+        solver.close()
+    return ranges
+
+
+def default_weights() -> Path:
+    return Path(__file__).resolve().parent / "tuned" / "surrogate_mlp.bin"
+
+
+def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, records: int = 12,
+                  samples: int = 4096, steps: int = 6000, batch: int = 8192, lr: float = 1e-3,
+                  log=sys.stderr) -> dict:
+    """The training recipe: `scenes` trajectories, the last `held_out` kept for evaluation;
+    Adam at lr for `steps` batches, then lr / 10 for a quarter as many."""
+    data = TrainingSet(scenes * records * samples)
+    ranges = generate(data, scenes, records, samples, log=log)
+    split = ranges[scenes - held_out][0] if held_out else len(data)
+    losses = net.train(data, 0, split, batch, steps, lr, losses=True)
+    tail = net.train(data, 0, split, batch, max(1, steps // 4), lr * 0.1, seed=11, losses=True)
+    report = {"rows": len(data), "train_rows": split, "loss_first": float(losses[0]),
+              "loss_last": float(tail[-1])}
+    if held_out:
+        report["held_out"] = net.evaluate(data, split, len(data))
+    report["train"] = net.evaluate(data, 0, min(split, 1 << 20))
+    data.close()
+    return report

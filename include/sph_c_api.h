@@ -301,6 +301,84 @@ int sph_batch_read_particles(SphBatch* batch, uint32_t scene, float* pos4, float
 int sph_batch_profile_enable(SphBatch* batch, int enable);
 int sph_batch_profile_read(SphBatch* batch, const char** names, double* ms, uint32_t* launches, uint32_t capacity);
 
+/* ---- Learned surrogate ----
+ * No counterpart in the reference: an addition on top of FluidSolver::step
+ * (src/physics/fluid_sph.cpp:151-303). A network predicts each particle's SPH
+ * acceleration (pressure + viscosity force / density) from its 8 nearest neighbours and
+ * holds it for 4 substeps; the density pass of the next substep checks the result and
+ * the plain path redoes any macro-step whose mean density excess is above a threshold.
+ *
+ * Network: 64 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 operands with FP32 accumulation,
+ * FP32 master weights, Adam. Parameters are W[out][in] then b[out] per layer (the last
+ * layer padded to 32 outputs, rows 3..31 held at zero); sph_net_param_count() floats.
+ * Feature row (64 BF16, bits as uint16): for the k-th nearest neighbour (k < 8),
+ * (dx, dy, dz) / h and (dvx, dvy, dvz) / (1 m/s) at 6k..6k+5; rho / rho0 - 1 at 48;
+ * neighbour-present flags at 49..56; h / (1 cm) at 57; zero at 58..63. Outputs and
+ * targets are accelerations in units of 9.81 m/s^2. */
+typedef struct SphNet SphNet;
+typedef struct SphDataset SphDataset;
+
+typedef struct SphRolloutStats {
+    uint64_t macroSteps;      /* network macro-steps attempted */
+    uint64_t fallbacks;       /* of those, redone with the plain path */
+    uint64_t plainSubsteps;   /* substeps run by the plain path (fallbacks and leftovers) */
+    float maxDeviation;       /* largest mean density excess seen by the check */
+    float lastDeviation;
+} SphRolloutStats;
+
+uint32_t sph_net_param_count(void);
+/* He-uniform weights from `seed` (splitmix64), zero biases. */
+int sph_net_create(int32_t device, uint64_t seed, SphNet** out);
+int sph_net_destroy(SphNet* net);
+int sph_net_get_params(SphNet* net, float* params);
+/* Sets the FP32 weights (frozen padding forced to zero) and resets Adam's state. */
+int sph_net_set_params(SphNet* net, const float* params);
+int sph_net_save(SphNet* net, const char* path);
+int sph_net_load(SphNet* net, const char* path);
+/* y[rows x 3] for feature rows x[rows x 64]. */
+int sph_net_forward(SphNet* net, const uint16_t* x, uint32_t rows, float* y);
+/* Gradient of the mean squared error (mean over rows x 3 outputs) w.r.t. the
+ * parameters, and the loss; no update. */
+int sph_net_gradients(SphNet* net, const uint16_t* x, const float* t, uint32_t rows, float* grad, float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on host rows. */
+int sph_net_train_batch(SphNet* net, const uint16_t* x, const float* t, uint32_t rows, float lr, float* loss);
+
+/* Device-resident training set of (feature row, target) pairs. */
+int sph_dataset_create(int32_t device, uint32_t capacity, SphDataset** out);
+int sph_dataset_destroy(SphDataset* data);
+uint32_t sph_dataset_size(const SphDataset* data);
+/* Records one macro-step of `solver`'s plain path: the features of `samples` distinct
+ * particles (seeded choice) at its start and, as targets, their mean SPH acceleration
+ * over the 4 following substeps of length dtSub. Advances the solver by 4 substeps. */
+int sph_dataset_record(SphDataset* data, SphSolver* solver, const SphParams* params, float dtSub, uint32_t samples,
+                       uint64_t seed);
+/* Rows [begin, begin + count): features x, targets t and, in t0, each particle's SPH
+ * acceleration of the first substep alone; any pointer may be NULL. */
+int sph_dataset_read(SphDataset* data, uint32_t begin, uint32_t count, uint16_t* x, float* t, float* t0);
+/* `steps` Adam steps on batches of batchRows rows drawn with replacement (seeded) from
+ * dataset rows [begin, end); losses[i] = loss of step i when losses is not NULL. */
+int sph_net_train(SphNet* net, SphDataset* data, uint32_t begin, uint32_t end, uint32_t batchRows, uint32_t steps,
+                  float lr, uint64_t seed, float* losses);
+/* Mean squared errors on rows [begin, end): out[0] of the network, out[1] of the
+ * no-network prediction (zero SPH acceleration: gravity alone), out[2] of holding the
+ * first substep's SPH acceleration for the macro-step (which needs a plain force pass). */
+int sph_net_evaluate(SphNet* net, SphDataset* data, uint32_t begin, uint32_t end, double out[3]);
+
+/* Feature rows of every particle of the solver's current state, in particle order. */
+int sph_features(SphSolver* solver, const SphParams* params, float dtSub, uint16_t* x);
+/* `steps` calls of FluidSolver::step(deltaSeconds) with the surrogate: every 4 substeps
+ * are one network macro-step checked by the next density pass; a macro-step whose mean
+ * max(rho / rho0 - 1, 0) exceeds `threshold` is redone by the plain path. Leftover
+ * substeps, and every substep of scenes with colliders, run plain. */
+int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* params, float deltaSeconds, uint32_t steps,
+                float threshold, SphRolloutStats* stats);
+/* Milliseconds spent in the network's forward passes inside sph_rollout since the last
+ * call, and the rows they covered; resets both. */
+int sph_net_inference_time(SphNet* net, double* ms, uint64_t* rows);
+/* out[0] = mean max(rho / rho0 - 1, 0), out[1] = mean |rho / rho0 - 1| of the current
+ * state, from a density pass of a substep of length dtSub. */
+int sph_density_deviation(SphSolver* solver, SphNet* net, const SphParams* params, float dtSub, double out[2]);
+
 #ifdef __cplusplus
 }
 #endif

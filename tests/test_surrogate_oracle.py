@@ -1,0 +1,202 @@
+"""Pins of the surrogate oracle (oracle/surrogate_oracle.py) on the CPU: BF16 rounding
+against torch, the network's backward pass against torch.autograd and finite
+differences, the BF16-mirroring mode against the exact one, Adam against its closed
+form, and the nearest-neighbour choice against scipy's kd-tree."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import surrogate_oracle as S
+
+F32 = np.float32
+
+
+def _net(seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(S.PARAMS)
+    for nin, nout, w, b in S.LAYERS:
+        bound = np.sqrt(6.0 / nin) * scale
+        p[w: w + nin * nout] = rng.uniform(-bound, bound, nin * nout)
+        p[b: b + nout] = rng.normal(0, 0.1, nout)
+    p[S.frozen_mask()] = 0.0
+    return p
+
+
+def _rows(n, seed=1):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 0.7, (n, S.FEATURES)).astype(F32)
+    x[:, 57:] = 0
+    return S.bf16(x), rng.normal(0, 0.5, (n, S.OUTPUTS)).astype(F32)
+
+
+def test_bf16_rounding_matches_torch():
+    rng = np.random.default_rng(3)
+    v = np.concatenate([
+        rng.normal(0, 1, 20000).astype(F32),
+        (rng.normal(0, 1, 2000) * 1e30).astype(F32),
+        (rng.normal(0, 1, 2000) * 1e-38).astype(F32),          # subnormals
+        np.array([0.0, -0.0, np.inf, -np.inf, 3.4e38, -3.4e38], dtype=F32),
+    ])
+    # exact ties: the dropped half is 0x8000, with even and odd kept parts
+    ties = (rng.integers(0, 1 << 15, 4000, dtype=np.uint32) << 16 | 0x8000).view(F32)
+    v = np.concatenate([v, ties[np.isfinite(ties)]])
+    ours = S.bf16(v)
+    theirs = torch.from_numpy(v).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(ours.view(np.uint32), theirs.view(np.uint32))
+    assert np.isnan(S.bf16(np.array([np.nan], dtype=F32)))[0]
+
+
+def _torch_loss(params, x, t):
+    p = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    a = torch.tensor(x, dtype=torch.float64)
+    for i, (nin, nout, w, b) in enumerate(S.LAYERS):
+        W = p[w: w + nin * nout].reshape(nout, nin)
+        z = a @ W.T + p[b: b + nout]
+        a = torch.relu(z) if i < 3 else z
+    y = a[:, : S.OUTPUTS]
+    loss = ((y - torch.tensor(t, dtype=torch.float64)) ** 2).mean()
+    loss.backward()
+    g = p.grad.numpy().copy()
+    g[S.frozen_mask()] = 0.0
+    return g, float(loss.detach())
+
+
+def test_exact_gradients_equal_torch_autograd():
+    p = _net(0)
+    x, t = _rows(96)
+    g, loss = S.gradients(p, x, t, mirror=False)
+    g_ref, loss_ref = _torch_loss(p, x, t)
+    assert loss == pytest.approx(loss_ref, rel=1e-12)
+    np.testing.assert_allclose(g, g_ref, rtol=1e-9, atol=1e-12)
+
+
+def test_exact_gradients_equal_central_differences():
+    p = _net(1)
+    x, t = _rows(40, seed=2)
+    g, _ = S.gradients(p, x, t, mirror=False)
+    rng = np.random.default_rng(5)
+    idx = []
+    for nin, nout, w, b in S.LAYERS:
+        live = S.OUTPUTS if nout == 32 else nout
+        idx += list(w + rng.integers(0, live * nin, 6)) + list(b + rng.integers(0, live, 2))
+    eps = 1e-6
+    for i in idx:
+        hi, lo = p.copy(), p.copy()
+        hi[i] += eps
+        lo[i] -= eps
+        fd = (S.gradients(hi, x, t)[1] - S.gradients(lo, x, t)[1]) / (2 * eps)
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-9), i
+
+
+class _RoundGrad(torch.autograd.Function):
+    """Identity forward; the gradient leaving it is rounded FP32 -> BF16."""
+    @staticmethod
+    def forward(ctx, z):
+        return z.clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        return g.to(torch.float32).to(torch.bfloat16).to(torch.float64)
+
+
+class _RoundValue(torch.autograd.Function):
+    """Values rounded FP32 -> BF16; the gradient passes unchanged."""
+    @staticmethod
+    def forward(ctx, a):
+        return a.to(torch.float32).to(torch.bfloat16).to(torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        return g
+
+
+def _torch_mirror_loss(params, x, t):
+    p = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    a = torch.tensor(x, dtype=torch.float64)
+    for i, (nin, nout, w, b) in enumerate(S.LAYERS):
+        W = _RoundValue.apply(p[w: w + nin * nout].reshape(nout, nin))
+        z = a @ W.T + p[b: b + nout]
+        if i < 3:
+            a = _RoundValue.apply(torch.relu(_RoundGrad.apply(z)))
+        else:
+            a = _RoundGrad.apply(z)
+    loss = ((a[:, : S.OUTPUTS] - torch.tensor(t, dtype=torch.float64)) ** 2).mean()
+    loss.backward()
+    g = p.grad.numpy().copy()
+    g[S.frozen_mask()] = 0.0
+    return g, float(loss.detach())
+
+
+def test_mirror_gradients_equal_torch_autograd_with_bf16_rounding():
+    p = _net(2)
+    x, t = _rows(256, seed=4)
+    g, loss = S.gradients(p, x, t, mirror=True)
+    g_ref, loss_ref = _torch_mirror_loss(p, x, t)
+    assert loss == pytest.approx(loss_ref, rel=1e-12)
+    # the output gradient is formed in FP32 on one side and FP64 on the other before its
+    # BF16 rounding: an occasional one-unit difference of a BF16 value
+    for nin, nout, w, b in S.LAYERS:
+        a, c = g_ref[w: w + nin * nout], g[w: w + nin * nout]
+        assert np.linalg.norm(c - a) / np.linalg.norm(a) < 1e-3
+
+
+def test_mirror_forward_stays_within_the_bf16_bound():
+    p = _net(2)
+    x, _ = _rows(256, seed=4)
+    y0, _ = S.forward(p, x, mirror=False)
+    y1, _ = S.forward(p, x, mirror=True)
+    # BF16 rounding of the weights and of three hidden layers, relative 2^-9 each
+    rel = np.linalg.norm(y1 - y0) / np.linalg.norm(y0)
+    assert 0 < rel < 8 * 2.0 ** -9
+
+
+def test_frozen_outputs_are_zero_and_carry_no_gradient():
+    p = _net(3)
+    x, t = _rows(32)
+    g, _ = S.gradients(p, x, t)
+    assert not np.any(g[S.frozen_mask()])
+    assert S.frozen_mask().sum() == (32 - 3) * 128 + (32 - 3)
+
+
+def test_adam_closed_form_for_a_constant_gradient():
+    w = np.array([1.0, -2.0, 0.5])
+    g = np.array([0.3, -1e-3, 2.0])
+    m1 = np.zeros(3)
+    m2 = np.zeros(3)
+    lr = 1e-2
+    for step in range(1, 51):
+        w, m1, m2 = S.adam(w, m1, m2, g, step, lr)
+    # with a constant gradient the bias-corrected moments are g and g^2 at every step
+    expect = np.array([1.0, -2.0, 0.5]) - 50 * lr * g / (np.abs(g) + 1e-8)
+    np.testing.assert_allclose(w, expect, rtol=1e-10, atol=1e-14)
+
+
+def test_knn_equals_a_kd_tree_query():
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(0, 1, (600, 3)).astype(F32)
+    h = 0.12
+    nbr = S.knn(pos, h)
+    tree = cKDTree(pos.astype(np.float64))
+    d, j = tree.query(pos.astype(np.float64), k=9, distance_upper_bound=h)
+    for i in range(len(pos)):
+        want = [int(u) for u, dd in zip(j[i], d[i]) if np.isfinite(dd) and u != i and dd < h][:8]
+        got = [int(u) for u in nbr[i] if u >= 0]
+        assert got == want, i
+        r = np.linalg.norm(pos[got] - pos[i], axis=1)
+        assert np.all(np.diff(r) >= 0) and np.all(r < h)
+
+
+def test_feature_row_layout():
+    pos = np.array([[0, 0, 0, 0], [0.01, 0, 0, 0], [0, 0.02, 0, 0], [1, 1, 1, 0]], dtype=F32)
+    vel = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 0, -2, 0], [0, 0, 0, 0]], dtype=F32)
+    rho = np.array([1010, 990, 1000, 1000], dtype=F32)
+    f = S.features(pos, vel, rho, 0.05, 1000.0)
+    assert f.shape == (4, 64)
+    np.testing.assert_allclose(f[0, 0:6], [0.2, 0, 0, 1, 0, 0], rtol=1e-6)
+    np.testing.assert_allclose(f[0, 6:12], [0, 0.4, 0, 0, 0, -2], rtol=1e-6)
+    assert list(f[0, 49:57]) == [1, 1, 0, 0, 0, 0, 0, 0]
+    assert f[0, 48] == F32(1010) / F32(1000) - 1
+    assert not np.any(f[3, :48]) and not np.any(f[3, 49:57])
+    assert np.all(f[:, 57] == F32(0.05) / F32(0.01))
+    assert not np.any(f[:, 58:])
