@@ -28,6 +28,7 @@ SOURCES = [
     "sph_engine.hip",
     "sph_api.hip",
     "sph_mlp.hip",
+    "sph_mlp_fused.hip",
     "sph_surrogate_kernels.hip",
     "sph_surrogate.hip",
 ]

@@ -151,6 +151,15 @@ __device__ __forceinline__ Where locate(const Launch& L) {
     return w;
 }
 
+// Extra outputs of the density pass for the learned surrogate, per slot t:
+//   slots[q * total + t]  the q-th nearest in-range neighbour by (r2, index), q < 8,
+//                         0xFFFFFFFF where fewer exist;
+//   aux[t]                (grad rho (x, y, z) from the poly6 gradient, in-range count).
+struct KnnOut {
+    uint32_t* slots;
+    float4* aux;
+};
+
 // launchers (sph_grid.hip); all batched launches use L.blocks workgroups
 hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t s);
 hipError_t launch_keys(const Launch& L, const GridBuffers& b, hipStream_t s);
@@ -168,7 +177,7 @@ hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hip
 
 // launchers (sph_forces.hip)
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, uint32_t* knn,
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, const KnnOut* knn,
                           const uint32_t* ids, hipStream_t s);
 hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
                         const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,

@@ -71,9 +71,9 @@ public:
 
     // ---- hooks of the learned surrogate ----
     // Grid build + density pass of one substep of length dt on the current state (with
-    // the 8 nearest neighbours per particle into knn[8 x total] when knn is set). The
-    // sorted state is left in the back buffers; nothing is committed.
-    int prepare(float dt, uint32_t* knn);
+    // the nearest-neighbour outputs of KnnOut when knn is set). The sorted state is left
+    // in the back buffers; nothing is committed.
+    int prepare(float dt, const sphdev::KnnOut* knn);
     float4* sorted_pos() { return pos[1 - pc]; }
     float4* sorted_vel() { return vel[1 - pc]; }
     uint32_t* sorted_ids() { return ids[1 - ic]; }
@@ -101,7 +101,7 @@ public:
 
 private:
     int ensure_plan(float dt, bool singleSubstep, uint32_t* maxSub);
-    int run_substep(uint32_t j, bool prepareOnly = false, uint32_t* knn = nullptr);
+    int run_substep(uint32_t j, bool prepareOnly = false, const sphdev::KnnOut* knn = nullptr);
     int enqueue_step(uint32_t maxSub);
     void invalidate();
     sphdev::Launch launch_for(uint32_t j) const;

@@ -469,7 +469,7 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     return SPH_OK;
 }
 
-int Engine::run_substep(uint32_t j, bool prepareOnly, uint32_t* knn) {
+int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::KnnOut* knn) {
     const sphdev::Launch L = launch_for(j);
     sphdev::GridBuffers g;
     g.pos = pos[pc];
@@ -567,7 +567,7 @@ int Engine::step(float dt, bool singleSubstep) {
     return SPH_OK;
 }
 
-int Engine::prepare(float dt, uint32_t* knn) {
+int Engine::prepare(float dt, const sphdev::KnnOut* knn) {
     if (int st = set_device()) return st;
     uint32_t maxSub = 0;
     if (int st = ensure_plan(dt, true, &maxSub)) return st;

@@ -76,12 +76,15 @@ __device__ __forceinline__ bool nearer(float r2, uint32_t id, float r2b, uint32_
 
 // With KNN the pass also keeps, per particle, the 8 nearest in-range candidates other
 // than itself by (r2, index) — a duplicate visit of a bucket cannot enter twice — and
-// writes their slots to knn[k * total + t] (0xFFFFFFFF where fewer than 8 exist).
+// writes their slots to knn[k * total + t] (0xFFFFFFFF where fewer than 8 exist), and
+// in aux[t] the density gradient of the poly6 estimate,
+// grad rho = -6 cPoly6 m sum (h^2 - r^2)^2 (x_t - x_u), summed in the density's order,
+// with the number of in-range candidates (duplicate visits included, self included).
 template <bool KNN>
 __global__ void __launch_bounds__(kBlock)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
               const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
-              uint32_t* __restrict__ knn, const uint32_t* __restrict__ knnIds) {
+              uint32_t* __restrict__ knn, float4* __restrict__ aux, const uint32_t* __restrict__ knnIds) {
     const Where w = locate(L);
     if (!w.valid || !w.d->active) return;
     const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
@@ -100,6 +103,7 @@ __global__ void __launch_bounds__(kBlock)
     uint4 group = make_uint4(0u, 0u, 0u, 0u);
     float kd[8];
     uint32_t ks[8], ki[8];
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         kd[q] = INFINITY;
@@ -150,6 +154,12 @@ __global__ void __launch_bounds__(kBlock)
                             if (lane4 == 3u) store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
                         }
                         ++listed;
+                        if (KNN) {
+                            const float d2 = d * d;
+                            gx += d2 * ddx;
+                            gy += d2 * ddy;
+                            gz += d2 * ddz;
+                        }
                         if (KNN && r2 <= kd[7]) {
                             const uint32_t u = lo + j + e;
                             bool dup = u == t;
@@ -188,6 +198,8 @@ __global__ void __launch_bounds__(kBlock)
     if (KNN) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) knn[(size_t)q * L.total + t] = ks[q];
+        const float gscale = (-6.0f * k.cPoly6) * k.mass;
+        aux[t] = make_float4(gx * gscale, gy * gscale, gz * gscale, (float)listed);
     }
 }
 
@@ -510,14 +522,15 @@ __global__ void __launch_bounds__(kBlock)
 }  // namespace
 
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, uint32_t* knn,
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, const KnnOut* knn,
                           const uint32_t* ids, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
     if (knn) {
-        k_density<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, knn, ids);
+        k_density<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, knn->slots,
+                                                    knn->aux, ids);
     } else {
         k_density<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, nullptr,
-                                                     nullptr);
+                                                     nullptr, nullptr);
     }
     return hipGetLastError();
 }

@@ -55,4 +55,11 @@ hipError_t launch_adam(float* w, float* m1, float* m2, const float* grad, uint32
                        float b2, float eps, float c1, float c2, uint16_t* wBf16, const float* frozen, hipStream_t s);
 hipError_t launch_to_bf16(const float* src, uint32_t count, uint16_t* dst, hipStream_t s);
 
+// Inference of the whole network in one kernel (sph_mlp_fused.hip): feature rows x
+// (rows x 64 BF16) -> y[row] = (out0, out1, out2, 0). wBf16 / wF32: the parameter
+// mirrors (W[out][in], b[out] per layer). blocks = 0 picks one workgroup per CU.
+hipError_t launch_mlp_fused(const uint16_t* x, uint32_t rows, const uint16_t* wBf16, const float* wF32, float4* y,
+                            uint32_t blocks, hipStream_t s);
+uint32_t mlp_fused_blocks(uint32_t rows);
+
 }  // namespace sphmlp

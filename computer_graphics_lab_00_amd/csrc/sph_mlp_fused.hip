@@ -1,0 +1,185 @@
+// sph_mlp_fused.hip — the surrogate network's inference pass in one kernel (gfx950).
+//
+// All four layers run back to back with the activations in registers: a workgroup
+// stages the BF16 weights (97 KB, rows padded so LDS reads are conflict free) and the
+// biases in LDS once, and each wave then streams groups of 64 feature rows through the
+// network. HBM traffic is one read of the 128-byte feature row and one 16-byte write of
+// the output per particle; the work is MFMA bound.
+//
+// Each layer computes the transposed product out^T = W * act^T with
+// v_mfma_f32_32x32x16_bf16: A = W (32 output features x 16 inputs, from LDS), B = act^T
+// (16 inputs x 32 particle rows), so lane l always owns particle row l % 32. The
+// accumulator of a 32 x 32 output tile holds, in lane l and element j, feature
+// 8 (j / 4) + 4 (l / 32) + j % 4; the next layer's B operand needs, in lane l and
+// element i of k-block kb, input 16 kb + 8 (l / 32) + i. The two orders are the same set
+// of 16 features per k-block, so instead of moving data between lanes the staged
+// weights of layers 1-3 are stored with their input columns permuted
+// (logical column 16 kb + 8 h + i holds physical input 16 kb + 8 (i / 4) + 4 h + i % 4)
+// and the accumulators are used as the next B operands in place.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_mlp.h"
+
+namespace sphmlp {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kWaves = 8;               // waves per workgroup, 2 per SIMD
+constexpr uint32_t kThreads = kWaves * 64;
+constexpr uint32_t kLd0 = 64 + 8;            // padded LDS row of layer 0 (64 inputs)
+constexpr uint32_t kLdH = 128 + 8;           // padded LDS row of layers 1-3 (128 inputs)
+constexpr uint32_t kOff1 = 128 * kLd0;
+constexpr uint32_t kOff2 = kOff1 + 128 * kLdH;
+constexpr uint32_t kOff3 = kOff2 + 128 * kLdH;
+constexpr uint32_t kLdsWeights = kOff3 + 32 * kLdH;   // BF16 elements
+constexpr uint32_t kBiases = 128 * 3 + 32;
+// parameter offsets (W[out][in] then b[out] per layer)
+constexpr uint32_t kW0 = 0, kB0 = 8192, kW1 = 8320, kB1 = 24704, kW2 = 24832, kB2 = 41216, kW3 = 41344,
+                   kB3 = 45440;
+
+__device__ __forceinline__ uint32_t physical_input(uint32_t k) {
+    const uint32_t r = k & 15u, h = r >> 3, i = r & 7u;
+    return (k & ~15u) + ((i >> 2) << 3) + (h << 2) + (i & 3u);
+}
+
+__device__ __forceinline__ __bf16 bits_bf16(uint16_t b) {
+    __bf16 r;
+    __builtin_memcpy(&r, &b, 2);
+    return r;
+}
+
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// Bias, ReLU and BF16 rounding of one 32 x 32 accumulator tile into two B operands.
+__device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias, uint32_t half, bf16x8& lo,
+                                            bf16x8& hi) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t f = 8u * (uint32_t)(j / 4) + 4u * half + (uint32_t)(j % 4);
+        float v = acc[j] + bias[f];
+        v = v > 0.0f ? v : 0.0f;
+        if (j < 8)
+            lo[j] = (__bf16)v;
+        else
+            hi[j - 8] = (__bf16)v;
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                             uint32_t lane, const bf16x8 (&in)[2][KB], bf16x8 (&out)[2][8]) {
+    const uint32_t r = lane & 31u, half = lane >> 5;
+    const uint32_t ld = KB == 4 ? kLd0 : kLdH;
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = f32x16{};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const bf16x8 a = lds8(w + (uint32_t)(mt * 32 + r) * ld + (uint32_t)kb * 16u + 8u * half);
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+                acc[nt][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, in[nt][kb], acc[nt][mt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) to_operands(acc[nt][mt], bias + mt * 32, half, out[nt][2 * mt], out[nt][2 * mt + 1]);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_mlp_fused(const uint16_t* __restrict__ x, uint32_t rows, const uint16_t* __restrict__ wb,
+                const float* __restrict__ wf, float4* __restrict__ y) {
+    __shared__ __attribute__((aligned(16))) uint16_t sw[kLdsWeights];
+    __shared__ float sb[kBiases];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < 128u * 64u; i += kThreads) sw[(i >> 6) * kLd0 + (i & 63u)] = wb[kW0 + i];
+    for (uint32_t i = tid; i < 128u * 128u; i += kThreads) {
+        const uint32_t o = i >> 7, k = i & 127u;
+        sw[kOff1 + o * kLdH + k] = wb[kW1 + o * 128u + physical_input(k)];
+        sw[kOff2 + o * kLdH + k] = wb[kW2 + o * 128u + physical_input(k)];
+    }
+    for (uint32_t i = tid; i < 32u * 128u; i += kThreads) {
+        const uint32_t o = i >> 7, k = i & 127u;
+        sw[kOff3 + o * kLdH + k] = wb[kW3 + o * 128u + physical_input(k)];
+    }
+    for (uint32_t i = tid; i < 128u; i += kThreads) {
+        sb[i] = wf[kB0 + i];
+        sb[128u + i] = wf[kB1 + i];
+        sb[256u + i] = wf[kB2 + i];
+    }
+    if (tid < 32u) sb[384u + tid] = wf[kB3 + tid];
+    __syncthreads();
+
+    const uint32_t lane = tid & 63u, r = lane & 31u, half = lane >> 5;
+    const uint32_t stride = gridDim.x * kWaves * 64u;
+    for (uint32_t base = (blockIdx.x * kWaves + (tid >> 6)) * 64u; base < rows; base += stride) {
+        // The weight fragments are loop invariant; without this the compiler hoists all
+        // of them into registers and spills. They are re-read from LDS per row group.
+        asm volatile("" ::: "memory");
+        bf16x8 in0[2][4];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const uint32_t row = base + (uint32_t)nt * 32u + r;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                if (row < rows) {
+                    in0[nt][kb] = *reinterpret_cast<const bf16x8*>(x + (size_t)row * 64u + (uint32_t)kb * 16u + 8u * half);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) in0[nt][kb][i] = bits_bf16(0);
+                }
+            }
+        }
+        bf16x8 h1[2][8], h2[2][8];
+        hidden_layer<4>(sw, sb, lane, in0, h1);
+        hidden_layer<8>(sw + kOff1, sb + 128, lane, h1, h2);
+        hidden_layer<8>(sw + kOff2, sb + 256, lane, h2, h1);
+        // output layer: 32 (3 used) outputs, one tile per row group
+        f32x16 acc[2] = {f32x16{}, f32x16{}};
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) {
+            const bf16x8 a = lds8(sw + kOff3 + r * kLdH + (uint32_t)kb * 16u + 8u * half);
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h1[nt][kb], acc[nt], 0, 0, 0);
+        }
+        if (half == 0u) {   // elements 0..2 of the lower half are outputs 0..2
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const uint32_t row = base + (uint32_t)nt * 32u + r;
+                if (row < rows)
+                    y[row] = make_float4(acc[nt][0] + sb[384], acc[nt][1] + sb[385], acc[nt][2] + sb[386], 0.0f);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+uint32_t mlp_fused_blocks(uint32_t rows) {
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        cus = 256;
+    const uint32_t groups = (rows + kWaves * 64u - 1u) / (kWaves * 64u);
+    const uint32_t cap = (uint32_t)(cus > 0 ? cus : 256);
+    return groups < cap ? (groups ? groups : 1u) : cap;
+}
+
+hipError_t launch_mlp_fused(const uint16_t* x, uint32_t rows, const uint16_t* wBf16, const float* wF32, float4* y,
+                            uint32_t blocks, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    if (blocks == 0) blocks = mlp_fused_blocks(rows);
+    k_mlp_fused<<<blocks, kThreads, 0, s>>>(x, rows, wBf16, wF32, y);
+    return hipGetLastError();
+}
+
+}  // namespace sphmlp

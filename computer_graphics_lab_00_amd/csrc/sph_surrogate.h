@@ -8,17 +8,18 @@
 
 namespace sphsur {
 
-constexpr uint32_t kFeatures = 64;   // 8 x 6 neighbour features, density, 8 presence flags, h, padding
+constexpr uint32_t kFeatures = 64;   // 8 x 7 neighbour features, density, its gradient, h, count, padding
 constexpr uint32_t kHidden = 128;
 constexpr uint32_t kOutPadded = 32;  // 3 outputs padded to one MFMA tile
 constexpr uint32_t kOutputs = 3;
 constexpr uint32_t kMacro = 4;       // substeps per surrogate macro-step
 constexpr float kVelocityScale = 1.0f;   // m/s
-constexpr float kAccelScale = 9.81f;     // m/s^2: targets and outputs are in units of |g|
+constexpr float kAccelScale = 98.1f;     // m/s^2: targets and outputs are in units of 10 |g|
+constexpr float kDensityScale = 100.0f;  // density deviations and gradients enter in percent
 constexpr float kLengthScale = 0.01f;    // m: the kernel-radius feature is h / 1 cm
 
 hipError_t launch_features(const sphdev::Launch& L, const float4* pos, const float4* vel, const uint32_t* knn,
-                           float vScale, uint16_t* x, hipStream_t s);
+                           const float4* aux, float vScale, uint16_t* x, hipStream_t s);
 hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, const float4* vel, const float* y,
                                   uint32_t ldY, float aScale, uint32_t substeps, float4* posOut, float4* velOut,
                                   hipStream_t s);

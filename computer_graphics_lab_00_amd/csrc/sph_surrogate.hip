@@ -69,6 +69,7 @@ struct SphNet {
     uint16_t *ix = nullptr, *ia = nullptr, *ib = nullptr;
     float* iy = nullptr;
     uint32_t* knn = nullptr;
+    float4* aux = nullptr;
     float2* devPartial = nullptr;
     uint32_t devPartialCap = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -81,7 +82,7 @@ struct SphNet {
         for (void* p : {(void*)w, (void*)m1, (void*)m2, (void*)grad, (void*)frozen, (void*)wb, (void*)x, (void*)h[0],
                         (void*)h[1], (void*)h[2], (void*)dy, (void*)dh[0], (void*)dh[1], (void*)y, (void*)t,
                         (void*)partial, (void*)rowErr, (void*)order, (void*)ix, (void*)ia, (void*)ib, (void*)iy,
-                        (void*)knn, (void*)devPartial})
+                        (void*)knn, (void*)aux, (void*)devPartial})
             hipFree(p);
         for (hipEvent_t e : ev)
             if (e) hipEventDestroy(e);
@@ -94,17 +95,18 @@ struct SphDataset {
     uint32_t cap = 0, size = 0;
     uint16_t* x = nullptr;
     float* t = nullptr;
-    float* t0 = nullptr;   // SPH acceleration of the first substep alone (units of |g|)
+    float* t0 = nullptr;   // SPH acceleration of the first substep alone (units of 10 g)
     // recording scratch, sized for the solver it records from
     uint32_t scratchN = 0;
     uint16_t* xAll = nullptr;
     uint32_t* knn = nullptr;
+    float4* aux = nullptr;
     float4* acc = nullptr;
     uint32_t *slots = nullptr, *sampleIds = nullptr;
 
     ~SphDataset() {
         hipSetDevice(device);
-        for (void* p : {(void*)x, (void*)t, (void*)t0, (void*)xAll, (void*)knn, (void*)acc, (void*)slots,
+        for (void* p : {(void*)x, (void*)t, (void*)t0, (void*)xAll, (void*)knn, (void*)aux, (void*)acc, (void*)slots,
                         (void*)sampleIds})
             hipFree(p);
     }
@@ -134,7 +136,7 @@ int net_ensure_batch(SphNet* n, uint32_t rowsPad) {
 
 int net_ensure_inference(SphNet* n, uint32_t rowsPad) {
     if (rowsPad <= n->icap) return SPH_OK;
-    for (void* p : {(void*)n->ix, (void*)n->ia, (void*)n->ib, (void*)n->iy, (void*)n->knn}) hipFree(p);
+    for (void* p : {(void*)n->ix, (void*)n->ia, (void*)n->ib, (void*)n->iy, (void*)n->knn, (void*)n->aux}) hipFree(p);
     const size_t R = rowsPad;
     SPH_HIP(hipMalloc(&n->ix, R * 64 * 2));
     SPH_HIP(hipMemset(n->ix, 0, R * 64 * 2));   // padded rows stay zero
@@ -142,6 +144,7 @@ int net_ensure_inference(SphNet* n, uint32_t rowsPad) {
     SPH_HIP(hipMalloc(&n->ib, R * 128 * 2));
     SPH_HIP(hipMalloc(&n->iy, R * 32 * 4));
     SPH_HIP(hipMalloc(&n->knn, R * 8 * 4));
+    SPH_HIP(hipMalloc(&n->aux, R * sizeof(float4)));
     n->icap = rowsPad;
     return SPH_OK;
 }
@@ -393,14 +396,14 @@ extern "C" int sph_net_forward(SphNet* n, const uint16_t* x, uint32_t rows, floa
     hipSetDevice(n->device);
     const uint32_t R = pad32(rows);
     if (int st = net_ensure_batch(n, R)) return st;
-    SPH_HIP(hipMemsetAsync(n->x, 0, (size_t)R * 64 * 2, n->stream));
     SPH_HIP(hipMemcpyAsync(n->x, x, (size_t)rows * 64 * 2, hipMemcpyHostToDevice, n->stream));
-    if (int st = net_forward(n, n->x, R, n->h[0], n->h[1], n->h[2], n->y, n->stream)) return st;
-    std::vector<float> yy((size_t)R * 32);
-    SPH_HIP(hipMemcpyAsync(yy.data(), n->y, (size_t)R * 32 * 4, hipMemcpyDeviceToHost, n->stream));
+    // the inference kernel of the rollout; training uses the layer-by-layer path
+    SPH_HIP(sphmlp::launch_mlp_fused(n->x, rows, n->wb, n->w, reinterpret_cast<float4*>(n->y), 0, n->stream));
+    std::vector<float> yy((size_t)rows * 4);
+    SPH_HIP(hipMemcpyAsync(yy.data(), n->y, (size_t)rows * 16, hipMemcpyDeviceToHost, n->stream));
     SPH_HIP(hipStreamSynchronize(n->stream));
     for (uint32_t r = 0; r < rows; ++r)
-        for (uint32_t c = 0; c < 3; ++c) y[(size_t)r * 3 + c] = yy[(size_t)r * 32 + c];
+        for (uint32_t c = 0; c < 3; ++c) y[(size_t)r * 3 + c] = yy[(size_t)r * 4 + c];
     return SPH_OK;
 }
 
@@ -468,7 +471,7 @@ extern "C" uint32_t sph_dataset_size(const SphDataset* d) { return d ? d->size :
 
 // Records one macro-step of the plain path: features of `samples` random particles at
 // its start, and as target each one's mean SPH acceleration (force / density, gravity
-// excluded) over the 4 substeps of length dtSub, in units of |g|; also the first
+// excluded) over the 4 substeps of length dtSub, in units of 10 g; also the first
 // substep's acceleration alone, the "hold the last acceleration" prediction. Advances
 // the solver by the 4 substeps.
 extern "C" int sph_dataset_record(SphDataset* d, SphSolver* solver, const SphParams* p, float dtSub,
@@ -483,18 +486,23 @@ extern "C" int sph_dataset_record(SphDataset* d, SphSolver* solver, const SphPar
     if (int st = e.ensure_force_buffer()) return st;
     const uint32_t N = e.total_slots();
     if (d->scratchN < N) {
-        for (void* q : {(void*)d->xAll, (void*)d->knn, (void*)d->acc, (void*)d->slots, (void*)d->sampleIds}) hipFree(q);
+        for (void* q : {(void*)d->xAll, (void*)d->knn, (void*)d->aux, (void*)d->acc, (void*)d->slots,
+                        (void*)d->sampleIds})
+            hipFree(q);
         SPH_HIP(hipMalloc(&d->xAll, (size_t)N * 64 * 2));
         SPH_HIP(hipMalloc(&d->knn, (size_t)N * 8 * 4));
+        SPH_HIP(hipMalloc(&d->aux, (size_t)N * 16));
         SPH_HIP(hipMalloc(&d->acc, (size_t)N * 16));
         SPH_HIP(hipMalloc(&d->slots, (size_t)N * 4));
         SPH_HIP(hipMalloc(&d->sampleIds, (size_t)N * 4));
         d->scratchN = N;
     }
     hipStream_t s = e.stream;
-    if (int st = e.prepare(dtSub, d->knn)) return st;
+    const sphdev::KnnOut ko{d->knn, d->aux};
+    if (int st = e.prepare(dtSub, &ko)) return st;
     const sphdev::Launch L = e.plan_launch();
-    SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), d->knn, sphsur::kVelocityScale, d->xAll, s));
+    SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), d->knn, d->aux, sphsur::kVelocityScale,
+                                    d->xAll, s));
     // `samples` distinct slots: a seeded partial Fisher-Yates shuffle.
     std::vector<uint32_t> perm(n);
     for (uint32_t i = 0; i < n; ++i) perm[i] = i;
@@ -634,14 +642,17 @@ extern "C" int sph_features(SphSolver* solver, const SphParams* p, float dtSub, 
     const uint32_t n = e.sc[0].n, N = e.total_slots();
     uint16_t *all = nullptr, *ordered = nullptr;
     uint32_t* knn = nullptr;
+    float4* aux = nullptr;
     SPH_HIP(hipMalloc(&all, (size_t)N * 64 * 2));
     SPH_HIP(hipMalloc(&ordered, (size_t)N * 64 * 2));
     SPH_HIP(hipMalloc(&knn, (size_t)N * 8 * 4));
-    int st = e.prepare(dtSub, knn);
+    SPH_HIP(hipMalloc(&aux, (size_t)N * 16));
+    const sphdev::KnnOut ko{knn, aux};
+    int st = e.prepare(dtSub, &ko);
     if (!st) {
         const sphdev::Launch L = e.plan_launch();
-        hipError_t he = sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), knn, sphsur::kVelocityScale, all,
-                                                e.stream);
+        hipError_t he = sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), knn, aux, sphsur::kVelocityScale,
+                                                all, e.stream);
         if (he == hipSuccess) he = sphsur::launch_unpermute_rows(all, e.sorted_ids(), n, ordered, e.stream);
         if (he == hipSuccess) he = hipMemcpyAsync(x, ordered, (size_t)n * 64 * 2, hipMemcpyDeviceToHost, e.stream);
         if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
@@ -650,6 +661,7 @@ extern "C" int sph_features(SphSolver* solver, const SphParams* p, float dtSub, 
     hipFree(all);
     hipFree(ordered);
     hipFree(knn);
+    hipFree(aux);
     return st;
 }
 
@@ -681,28 +693,29 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
     if (int rc = net_ensure_inference(net, R)) return rc;
     hipStream_t s = e.stream;
     bool prepared = false;
+    const sphdev::KnnOut ko{net->knn, net->aux};
     for (uint32_t step = 0; step < steps; ++step) {
         for (uint32_t m = 0; m < macros; ++m) {
             if (!prepared) {
-                if (int rc = e.prepare(dtSub, net->knn)) return rc;
+                if (int rc = e.prepare(dtSub, &ko)) return rc;
             }
             const sphdev::Launch L = e.plan_launch();
-            SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), net->knn, sphsur::kVelocityScale,
-                                            net->ix, s));
+            SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), net->knn, net->aux,
+                                            sphsur::kVelocityScale, net->ix, s));
             SPH_HIP(hipEventRecord(net->ev[0], s));
-            if (int rc = net_forward(net, net->ix, R, net->ia, net->ib, net->ia, net->iy, s)) return rc;
+            SPH_HIP(sphmlp::launch_mlp_fused(net->ix, n, net->wb, net->w, reinterpret_cast<float4*>(net->iy), 0, s));
             SPH_HIP(hipEventRecord(net->ev[1], s));
             if (int rc = e.backup_state()) return rc;
-            SPH_HIP(sphsur::launch_integrate_macro(L, e.sorted_pos(), e.sorted_vel(), net->iy, 32, sphsur::kAccelScale,
+            SPH_HIP(sphsur::launch_integrate_macro(L, e.sorted_pos(), e.sorted_vel(), net->iy, 4, sphsur::kAccelScale,
                                                    sphsur::kMacro, e.state_pos(), e.state_vel(), s));
             e.commit_sorted();
-            if (int rc = e.prepare(dtSub, net->knn)) return rc;
+            if (int rc = e.prepare(dtSub, &ko)) return rc;
             double dev[2];
             if (int rc = deviation_of(net, e, dev)) return rc;
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, net->ev[0], net->ev[1]) == hipSuccess) {
                 net->mlpMs += ms;
-                net->mlpRows += R;
+                net->mlpRows += n;
             }
             st.macroSteps += 1;
             st.lastDeviation = (float)dev[0];

@@ -1,8 +1,8 @@
 // sph_surrogate_kernels.hip — particle-side kernels of the learned surrogate (gfx950).
 //
 // The surrogate replaces the force pass of four substeps with one network evaluation:
-// per particle, 64 features of its 8 nearest neighbours (from the density pass of the
-// grid) go through the MLP, the predicted SPH acceleration is held for a macro-step of 4
+// per particle, 64 features of its 8 nearest neighbours and of its density field (from
+// the density pass of the grid) go through the MLP, the predicted SPH acceleration is held for a macro-step of 4
 // substeps of symplectic Euler with the reference's speed clamp and walls
 // (src/physics/fluid_sph.cpp:260-285), and the next density pass checks the result.
 // Scenes with colliders are not handed to the surrogate: they run the plain path.
@@ -25,18 +25,28 @@ namespace {
 __device__ inline float clampf_ref(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
 // Features of particle t (slot order of the prepared state). Row layout, 64 BF16:
-// [8 x (dx, dy, dz) / h, (dvx, dvy, dvz) / vScale] | rho / rho0 - 1 | 8 presence flags |
-// h / kLengthScale | 0 x 6. The kernel radius lets one network serve scenes of several
-// resolutions (the SPH acceleration scales with 1 / h at equal compression).
+//   7q .. 7q+6: the q-th nearest neighbour (q < 8): (dx, dy, dz) / h, (dvx, dvy, dvz) /
+//               vScale, its 100 (rho / rho0 - 1) (zeros where absent)
+//   56        : own 100 (rho / rho0 - 1)
+//   57 .. 59  : 100 grad rho * h / rho0 (the poly6 gradient of the density pass)
+// Density deviations are a few percent at most, so they enter in percent to be of the
+// same order as the other features.
+//   60        : h / kLengthScale (one network serves several resolutions: at equal
+//               compression the SPH acceleration scales with 1 / h)
+//   61        : in-range candidates of the density pass / 32
+//   62, 63    : 0
+// Offsets are relative to the particle, so the features are translation invariant.
 __global__ void __launch_bounds__(kBlock)
     k_features(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
-               const uint32_t* __restrict__ knn, float vScale, uint16_t* __restrict__ x) {
+               const uint32_t* __restrict__ knn, const float4* __restrict__ aux, float vScale,
+               uint16_t* __restrict__ x) {
     const Where w = locate(L);
     if (!w.valid) return;
     const StepConsts k = w.d->k;
     const uint32_t t = w.slot;
-    const float4 p = pos[t], v = vel[t];
-    const float invH = 1.0f / k.h, invV = 1.0f / vScale;
+    const float4 p = pos[t], v = vel[t], a = aux[t];
+    const float invH = 1.0f / k.h, invV = 1.0f / vScale, invRho = 1.0f / k.restDensity;
+    const float ds = kDensityScale;
     float f[kFeatures];
 #pragma unroll
     for (int i = 0; i < kFeatures; ++i) f[i] = 0.0f;
@@ -45,17 +55,22 @@ __global__ void __launch_bounds__(kBlock)
         const uint32_t u = knn[(size_t)q * L.total + t];
         if (u != 0xFFFFFFFFu) {
             const float4 pu = pos[u], vu = vel[u];
-            f[6 * q + 0] = (pu.x - p.x) * invH;
-            f[6 * q + 1] = (pu.y - p.y) * invH;
-            f[6 * q + 2] = (pu.z - p.z) * invH;
-            f[6 * q + 3] = (vu.x - v.x) * invV;
-            f[6 * q + 4] = (vu.y - v.y) * invV;
-            f[6 * q + 5] = (vu.z - v.z) * invV;
-            f[49 + q] = 1.0f;
+            f[7 * q + 0] = (pu.x - p.x) * invH;
+            f[7 * q + 1] = (pu.y - p.y) * invH;
+            f[7 * q + 2] = (pu.z - p.z) * invH;
+            f[7 * q + 3] = (vu.x - v.x) * invV;
+            f[7 * q + 4] = (vu.y - v.y) * invV;
+            f[7 * q + 5] = (vu.z - v.z) * invV;
+            f[7 * q + 6] = (pu.w * invRho - 1.0f) * ds;
         }
     }
-    f[48] = p.w / k.restDensity - 1.0f;
-    f[57] = k.h / kLengthScale;
+    f[56] = (p.w * invRho - 1.0f) * ds;
+    const float gs = (k.h * invRho) * ds;
+    f[57] = a.x * gs;
+    f[58] = a.y * gs;
+    f[59] = a.z * gs;
+    f[60] = k.h / kLengthScale;
+    f[61] = a.w * (1.0f / 32.0f);
     uint4* row = reinterpret_cast<uint4*>(x + (size_t)t * kFeatures);
 #pragma unroll
     for (int c = 0; c < kFeatures / 8; ++c) {
@@ -248,10 +263,10 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
-hipError_t launch_features(const Launch& L, const float4* pos, const float4* vel, const uint32_t* knn, float vScale,
-                           uint16_t* x, hipStream_t s) {
+hipError_t launch_features(const Launch& L, const float4* pos, const float4* vel, const uint32_t* knn,
+                           const float4* aux, float vScale, uint16_t* x, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_features<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, knn, vScale, x);
+    k_features<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, knn, aux, vScale, x);
     return hipGetLastError();
 }
 

@@ -232,7 +232,8 @@ class TrainingScene:
 def training_scene(index: int, seed: int = 2024) -> TrainingScene:
     """Scene `index` of the seeded training family: one of four 64k lattices at one of the
     two resolutions, placed at a random spot of a box 1.2-2x its size (resting on the
-    floor half of the time), with velocities from N(0, 0.5 m/s)."""
+    floor half of the time), with velocities from N(0, 0.5 m/s), recorded after 0-47
+    plain steps so collapsing and splashing states are covered."""
     u = (splitmix64(seed * 1000003 + index, 8) >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
     lattice = LATTICES[index % len(LATTICES)]
     h = RADII[(index // len(LATTICES)) % len(RADII)]
@@ -251,14 +252,33 @@ def training_scene(index: int, seed: int = 2024) -> TrainingScene:
     wl = Workload(f"train_{index}", lattice, spacing, (0.0, 0.0, 0.0), tuple(box), h,
                   cell_count=max(1024, 1 << int(math.ceil(math.log2(2 * count)))),
                   jitter_seed=(seed + index) & 0xFFFFFFFF, corner=tuple(corner))
-    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * 12))
+    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * 48))
+
+
+# Position noise (fraction of the lattice spacing) and velocity noise (m/s) added before
+# the records of a scene, cycled record by record: the network also sees states a little
+# off the plain trajectory, with the plain path's response to them as the target, so a
+# rollout that drifts is pulled back instead of running away (noise injection,
+# Sanchez-Gonzalez et al. 2020).
+NOISE = ((0.0, 0.0), (0.005, 0.02), (0.01, 0.05), (0.02, 0.1))
+
+
+def perturb(solver: FluidSolver, spacing: float, pos_noise: float, vel_noise: float, seed: int) -> None:
+    if pos_noise <= 0.0 and vel_noise <= 0.0:
+        return
+    pos, vel = solver.state()
+    n = pos.shape[0]
+    pos[:, :3] += gaussian(seed, 3 * n, pos_noise * spacing).reshape(n, 3)
+    vel[:, :3] += gaussian(seed ^ 0x5DEECE66D, 3 * n, vel_noise).reshape(n, 3)
+    solver.load_state(pos, vel)
 
 
 def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4096, dt: float = 1.0 / 600.0,
-             gap_steps: int = 1, seed: int = 2024, first: int = 0, log=None) -> list[tuple[int, int]]:
+             gap_steps: int = 2, seed: int = 2024, first: int = 0, noise=NOISE, log=None) -> list[tuple[int, int]]:
     """Records `records` macro-steps of each of scenes first..first+scenes-1 into `data`;
     returns each scene's row range. Between records the scene runs `gap_steps` plain
-    steps so the rows of one scene are not all alike."""
+    steps so the rows of one scene are not all alike; before record r the state is
+    perturbed by noise[r % len(noise)]."""
     ranges = []
     solver = None
     cap = 0
@@ -278,6 +298,9 @@ def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4
             solver.step(p, dt)
         begin = len(data)
         for r in range(records):
+            if noise:
+                pn, vn = noise[r % len(noise)]
+                perturb(solver, sc.workload.spacing, pn, vn, seed=(seed << 24) + i * 1000 + r)
             data.record(solver, p, dt_sub, min(samples, n), seed=(seed << 20) + i * 1000 + r)
             for _ in range(gap_steps):
                 solver.step(p, dt)
@@ -293,20 +316,197 @@ def default_weights() -> Path:
     return Path(__file__).resolve().parent / "tuned" / "surrogate_mlp.bin"
 
 
+def record_workload(data: TrainingSet, workload: Workload, records: int, samples: int, warmup_steps: int = 0,
+                    gap_steps: int = 2, dt: float = 1.0 / 600.0, seed: int = 99) -> tuple[int, int]:
+    """Rows recorded from any workload's plain trajectory (e.g. the dam breaks the
+    rollout is measured on), for evaluating the network on that distribution."""
+    p = workload.params()
+    pos, vel = workload.initial_state(p)
+    n = pos.shape[0]
+    solver = FluidSolver(n, workload.cell_count, keep_forces=True)
+    solver.load_state(pos, vel)
+    sub = fluid_substeps(p, dt)
+    dt_sub = float(np.float32(np.float32(min(dt, 1.0 / 30.0)) / np.float32(sub)))
+    for _ in range(warmup_steps):
+        solver.step(p, dt)
+    begin = len(data)
+    for r in range(records):
+        data.record(solver, p, dt_sub, min(samples, n), seed=seed * 1000 + r)
+        for _ in range(gap_steps):
+            solver.step(p, dt)
+    solver.close()
+    return begin, len(data)
+
+
 def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, records: int = 12,
                   samples: int = 4096, steps: int = 6000, batch: int = 8192, lr: float = 1e-3,
-                  log=sys.stderr) -> dict:
+                  eval_workloads=(), log=sys.stderr) -> dict:
     """The training recipe: `scenes` trajectories, the last `held_out` kept for evaluation;
-    Adam at lr for `steps` batches, then lr / 10 for a quarter as many."""
-    data = TrainingSet(scenes * records * samples)
+    Adam at lr for `steps` batches, then lr / 10 for a quarter as many. Each workload in
+    `eval_workloads` contributes 8 records of 16384 rows from its own trajectory, used for
+    evaluation only."""
+    data = TrainingSet(scenes * records * samples + len(eval_workloads) * 8 * 16384)
     ranges = generate(data, scenes, records, samples, log=log)
     split = ranges[scenes - held_out][0] if held_out else len(data)
+    end = len(data)
+    evals = [(wl.name, record_workload(data, wl, 8, 16384, warmup_steps=4, gap_steps=4)) for wl in eval_workloads]
     losses = net.train(data, 0, split, batch, steps, lr, losses=True)
     tail = net.train(data, 0, split, batch, max(1, steps // 4), lr * 0.1, seed=11, losses=True)
     report = {"rows": len(data), "train_rows": split, "loss_first": float(losses[0]),
               "loss_last": float(tail[-1])}
     if held_out:
-        report["held_out"] = net.evaluate(data, split, len(data))
+        report["held_out"] = net.evaluate(data, split, end)
     report["train"] = net.evaluate(data, 0, min(split, 1 << 20))
+    for name, (b0, b1) in evals:
+        report["eval_" + name] = net.evaluate(data, b0, b1)
     data.close()
     return report
+
+
+# ---- evaluation: plain path against the surrogate on one scene ----
+
+def _energy(pos, vel, params: SphParams) -> float:
+    """Sum of 1/2 m |v|^2 - m g.x in FP64 (the conservation sum of the plain path)."""
+    m = float(params.particleMass)
+    g = np.array(params.gravity, np.float64)
+    v = vel[:, :3].astype(np.float64)
+    x = pos[:, :3].astype(np.float64)
+    return float(0.5 * m * np.sum(v * v) - m * np.sum(x @ g))
+
+
+def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 / 600.0,
+            threshold: float = DEFAULT_THRESHOLD, stats_every: int = 1, log=None) -> dict:
+    """Runs `steps` FluidSolver.step(dt) calls of `workload` with the plain path and with
+    the surrogate from the same initial state and returns timing and density/energy
+    statistics. The timed runs carry no statistics work; the statistics come from
+    separate, untimed runs."""
+    import time
+    p = workload.params()
+    pos0, vel0 = workload.initial_state(p)
+    n = pos0.shape[0]
+    sub = fluid_substeps(p, dt)
+    dt_sub = float(np.float32(np.float32(min(dt, 1.0 / 30.0)) / np.float32(sub)))
+    solver = FluidSolver(n, workload.cell_count)
+    result = {"particles": n, "steps": steps, "substeps_per_step": sub, "threshold": threshold}
+
+    def run(use_net: bool, timed: bool):
+        nonlocal threshold
+        solver.load_state(pos0, vel0)
+        st = SphRolloutStats()
+        series = []
+        solver.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            if use_net:
+                rollout(solver, net, p, dt, 1, threshold, st)
+            else:
+                solver.step(p, dt)
+            if not timed and (i + 1) % stats_every == 0:
+                series.append(density_deviation(solver, net, p, dt_sub))
+        solver.synchronize()
+        return time.perf_counter() - t0, st, series
+
+    run(False, True)   # warm-up: plans, graphs
+    t_plain, _, _ = run(False, True)
+    net.inference_time()
+    t_sur, st, _ = run(True, True)
+    mlp_ms, mlp_rows = net.inference_time()
+    # the same rollout with the check never failing: the cost of the surrogate alone
+    saved = threshold
+    threshold = float("inf")
+    t_free, _, _ = run(True, True)
+    net.inference_time()
+    result["unchecked_ms_per_step"] = 1e3 * t_free / steps
+    result["unchecked_speedup"] = t_plain / t_free
+    _, _, free_series = run(True, False)
+    result["unchecked_density_excess"] = [float(v[0]) for v in free_series]
+    threshold = saved
+    result["plain_ms_per_step"] = 1e3 * t_plain / steps
+    result["surrogate_ms_per_step"] = 1e3 * t_sur / steps
+    result["plain_particle_steps_per_s"] = n * steps / t_plain
+    result["surrogate_particle_steps_per_s"] = n * steps / t_sur
+    result["speedup"] = t_plain / t_sur
+    result["macro_steps"] = int(st.macroSteps)
+    result["fallbacks"] = int(st.fallbacks)
+    result["fallback_rate"] = st.fallbacks / max(1, st.macroSteps)
+    result["mlp_ms"] = mlp_ms
+    result["mlp_rows"] = mlp_rows
+    if mlp_ms > 0:
+        flop = 2.0 * sum(nin * nout for nin, nout in ((64, 128), (128, 128), (128, 128), (128, 32))) * mlp_rows
+        result["mlp_tflops"] = flop / (mlp_ms * 1e-3) / 1e12
+    if log is not None:
+        print(f"[surrogate] timed: plain {result['plain_ms_per_step']:.3f} ms/step, surrogate "
+              f"{result['surrogate_ms_per_step']:.3f} ms/step, fallbacks {st.fallbacks}/{st.macroSteps}",
+              file=log, flush=True)
+    _, _, plain_series = run(False, False)
+    e_plain = _energy(*solver.state(), p)
+    _, st2, sur_series = run(True, False)
+    e_sur = _energy(*solver.state(), p)
+    e0 = _energy(pos0, vel0, p)
+    ps = np.array(plain_series)
+    ss = np.array(sur_series)
+    result["plain_density_excess_mean"] = float(ps[:, 0].mean())
+    result["plain_density_abs_mean"] = float(ps[:, 1].mean())
+    result["surrogate_density_excess_mean"] = float(ss[:, 0].mean())
+    result["surrogate_density_abs_mean"] = float(ss[:, 1].mean())
+    result["density_error_vs_plain"] = float(np.mean(np.abs(ss[:, 1] - ps[:, 1])))
+    result["energy_initial"] = e0
+    result["energy_drift_plain"] = (e_plain - e0) / abs(e0)
+    result["energy_drift_surrogate"] = (e_sur - e0) / abs(e0)
+    result["series_plain"] = ps.tolist()
+    result["series_surrogate"] = ss.tolist()
+    solver.close()
+    return result
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m computer_graphics_lab_00_amd.surrogate")
+    sp = ap.add_subparsers(dest="cmd", required=True)
+    t = sp.add_parser("train", help="record trajectories and train the network on the GPU")
+    t.add_argument("--scenes", type=int, default=200)
+    t.add_argument("--held-out", type=int, default=20)
+    t.add_argument("--records", type=int, default=12)
+    t.add_argument("--samples", type=int, default=4096)
+    t.add_argument("--steps", type=int, default=6000)
+    t.add_argument("--batch", type=int, default=8192)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=1)
+    t.add_argument("--out", default=str(default_weights()))
+    t.add_argument("--eval", nargs="*", default=["m1"], choices=("m1", "r4m", "s16k"),
+                   help="dam breaks whose own trajectories the network is also evaluated on")
+    e = sp.add_parser("compare", help="plain path against the surrogate on a dam break")
+    e.add_argument("--weights", default=str(default_weights()))
+    e.add_argument("--workload", choices=("m1", "r4m", "s16k"), default="m1")
+    e.add_argument("--steps", type=int, default=20)
+    e.add_argument("--threshold", type=float, default=DEFAULT_THRESHOLD)
+    e.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    if a.cmd == "train":
+        import time
+        net = SurrogateNet(seed=a.seed)
+        t0 = time.perf_counter()
+        from . import fluid
+        wls = [{"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[w]() for w in a.eval]
+        rep = train_default(net, a.scenes, a.held_out, a.records, a.samples, a.steps, a.batch, a.lr, wls)
+        rep["seconds"] = time.perf_counter() - t0
+        Path(a.out).parent.mkdir(parents=True, exist_ok=True)
+        net.save(a.out)
+        print(json.dumps(rep), flush=True)
+        return 0
+    from . import fluid
+    net = SurrogateNet()
+    net.load(a.weights)
+    wl = {"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[a.workload]()
+    res = compare(wl, net, a.steps, threshold=a.threshold, log=sys.stderr)
+    if a.json:
+        Path(a.json).write_text(json.dumps(res))
+    res.pop("series_plain")
+    res.pop("series_surrogate")
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -311,10 +311,13 @@ int sph_batch_profile_read(SphBatch* batch, const char** names, double* ms, uint
  * Network: 64 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 operands with FP32 accumulation,
  * FP32 master weights, Adam. Parameters are W[out][in] then b[out] per layer (the last
  * layer padded to 32 outputs, rows 3..31 held at zero); sph_net_param_count() floats.
- * Feature row (64 BF16, bits as uint16): for the k-th nearest neighbour (k < 8),
- * (dx, dy, dz) / h and (dvx, dvy, dvz) / (1 m/s) at 6k..6k+5; rho / rho0 - 1 at 48;
- * neighbour-present flags at 49..56; h / (1 cm) at 57; zero at 58..63. Outputs and
- * targets are accelerations in units of 9.81 m/s^2. */
+ * Feature row (64 BF16, bits as uint16): for the q-th nearest in-range neighbour
+ * (q < 8, by distance then index) at 7q..7q+6: (dx, dy, dz) / h, (dvx, dvy, dvz) /
+ * (1 m/s) and its 100 (rho / rho0 - 1) (zeros where absent); at 56 the particle's own
+ * 100 (rho / rho0 - 1); at 57..59 100 grad rho * h / rho0, the gradient of the poly6
+ * density estimate; at 60 h / (1 cm); at 61 the density pass's in-range candidate
+ * count / 32; zero at 62, 63. Outputs and targets are accelerations in units of
+ * 98.1 m/s^2 (10 g). */
 typedef struct SphNet SphNet;
 typedef struct SphDataset SphDataset;
 

@@ -7,9 +7,16 @@ restatement of this project's own definitions, pinned as follows:
 
   bf16()            round-to-nearest-even FP32 -> BF16; pinned against torch's
                     independent conversion (tests/test_surrogate_oracle.py).
-  knn()/features()  brute-force 8 nearest in-range neighbours by (r2, index), FP32
+  knn()             brute-force 8 nearest in-range neighbours by (r2, index), FP32
                     arithmetic in the order the density pass uses; pinned by invariants
                     (sorted distances, in range, no self) and by the kd-tree of scipy.
+  walk()            the density pass's neighbour walk (27 buckets in dz, dy, dx order,
+                    members by index, 32 per bucket, duplicate visits kept —
+                    src/physics/fluid_sph.cpp:194-223) with the surrogate's extra sums:
+                    the poly6 density gradient and the in-range count. Pinned by its
+                    density equalling the FP32 SPH oracle's bit for bit and its gradient
+                    equalling central differences of the FP64 density estimate.
+  features()        the feature row from the walk and knn outputs.
   forward()/gradients(), exact mode (FP64, no rounding): pinned against torch.autograd in
                     float64 and central finite differences.
   forward()/gradients(), mirror mode: the same, with BF16 rounding where the GPU rounds
@@ -32,7 +39,8 @@ PARAMS = 45472
 OUTPUTS = 3
 FEATURES = 64
 MACRO = 4
-ACCEL_SCALE = 9.81
+ACCEL_SCALE = 98.1
+DENSITY_SCALE = 100.0
 VELOCITY_SCALE = 1.0
 LENGTH_SCALE = 0.01
 
@@ -150,30 +158,93 @@ def knn(pos3, h):
     return out
 
 
-def features(pos4, vel4, density, h, rest_density, nbr=None, v_scale=VELOCITY_SCALE):
+def _hash(cells, cell_count):
+    c = cells.astype(np.int64) & 0xFFFFFFFF
+    h = ((c[:, 0] * 73856093) & 0xFFFFFFFF) ^ ((c[:, 1] * 19349663) & 0xFFFFFFFF) ^ \
+        ((c[:, 2] * 83492791) & 0xFFFFFFFF)
+    return h & (cell_count - 1)
+
+
+def walk(pos4, h, cell_count, mass, c_poly6, cap=32):
+    """(density, grad rho (n x 3), in-range count) of the density pass, summed in FP32 in
+    the pass's visiting order. c_poly6 must be the pass's FP32 constant."""
+    p = np.asarray(pos4, dtype=F32)[:, :3]
+    n = p.shape[0]
+    hf = F32(h)
+    cells = np.floor(p / hf).astype(np.int64)
+    buckets = _hash(cells, cell_count)
+    order = np.lexsort((np.arange(n), buckets))
+    members = {}
+    for b in np.unique(buckets):
+        members[int(b)] = []
+    for i in order:
+        lst = members[int(buckets[i])]
+        if len(lst) < cap:
+            lst.append(int(i))
+    offsets = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    cands = []
+    for i in range(n):
+        lst = []
+        for off in offsets:
+            b = int(_hash((cells[i] + np.array(off))[None, :], cell_count)[0])
+            lst.extend(members.get(b, ()))
+        cands.append(lst)
+    width = max(len(c) for c in cands)
+    cand = np.full((n, width), -1, dtype=np.int64)
+    for i, c in enumerate(cands):
+        cand[i, : len(c)] = c
+    h2 = hf * hf
+    m = F32(mass)
+    c6 = F32(c_poly6)
+    rho = np.zeros(n, F32)
+    g = np.zeros((n, 3), F32)
+    listed = np.zeros(n, np.int64)
+    for c in range(width):
+        u = cand[:, c]
+        valid = u >= 0
+        q = p[np.where(valid, u, 0)]
+        d3 = p - q
+        r2 = (d3[:, 0] * d3[:, 0] + d3[:, 1] * d3[:, 1]) + d3[:, 2] * d3[:, 2]
+        inr = valid & (r2 < h2)
+        d = h2 - r2
+        rho = np.where(inr, rho + m * (((c6 * d) * d) * d), rho)
+        d2 = d * d
+        g = np.where(inr[:, None], g + d2[:, None] * d3, g)
+        listed += inr
+    rho = np.where(rho < F32(1e-6), F32(1e-6), rho)
+    gscale = (F32(-6.0) * c6) * m
+    return rho, g * gscale, listed
+
+
+def features(pos4, vel4, density, grad, listed, h, rest_density, nbr=None, v_scale=VELOCITY_SCALE):
     """Feature rows (n x 64, FP32 values before BF16 rounding) of every particle."""
     pos = np.asarray(pos4, dtype=F32)[:, :3]
     vel = np.asarray(vel4, dtype=F32)[:, :3]
+    rho = np.asarray(density, dtype=F32)
     if nbr is None:
         nbr = knn(pos, h)
     n = pos.shape[0]
     inv_h = F32(1.0) / F32(h)
     inv_v = F32(1.0) / F32(v_scale)
+    inv_rho = F32(1.0) / F32(rest_density)
     f = np.zeros((n, FEATURES), dtype=F32)
     for q in range(8):
         u = nbr[:, q]
         has = u >= 0
         uu = np.where(has, u, 0)
-        f[:, 6 * q: 6 * q + 3] = np.where(has[:, None], (pos[uu] - pos) * inv_h, 0)
-        f[:, 6 * q + 3: 6 * q + 6] = np.where(has[:, None], (vel[uu] - vel) * inv_v, 0)
-        f[:, 49 + q] = has.astype(F32)
-    f[:, 48] = np.asarray(density, dtype=F32) / F32(rest_density) - F32(1.0)
-    f[:, 57] = F32(h) / F32(LENGTH_SCALE)
+        f[:, 7 * q: 7 * q + 3] = np.where(has[:, None], (pos[uu] - pos) * inv_h, 0)
+        f[:, 7 * q + 3: 7 * q + 6] = np.where(has[:, None], (vel[uu] - vel) * inv_v, 0)
+        f[:, 7 * q + 6] = np.where(has, (rho[uu] * inv_rho - F32(1.0)) * F32(DENSITY_SCALE), 0)
+    f[:, 56] = (rho * inv_rho - F32(1.0)) * F32(DENSITY_SCALE)
+    gs = (F32(h) * inv_rho) * F32(DENSITY_SCALE)
+    f[:, 57:60] = np.asarray(grad, dtype=F32) * gs
+    f[:, 60] = F32(h) / F32(LENGTH_SCALE)
+    f[:, 61] = np.asarray(listed).astype(F32) * F32(1.0 / 32.0)
     return f
 
 
 def targets(forces_and_density):
-    """Mean over the substeps of force / density, in units of 9.81 m/s^2; the input is a
+    """Mean over the substeps of force / density, in units of 98.1 m/s^2; the input is a
     list of (force (n x 3), density (n,)) per substep, indexed by particle."""
     acc = None
     for f, rho in forces_and_density:

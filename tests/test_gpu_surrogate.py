@@ -167,16 +167,28 @@ def _oracle_density(oracle_lib, p, pos, vel, dt_sub):
     return o.particles()[:, 3].copy(), o
 
 
+def _oracle_features(p, pos, vel, dt_sub, rho_check=None):
+    consts = (C.c_float * 8)()
+    capi.check(capi.load().sph_debug_step_constants(C.byref(p), dt_sub, consts))
+    rho, grad, listed = SO.walk(pos, p.smoothingRadius, p.cellCount, p.particleMass, consts[2])
+    if rho_check is not None:
+        assert np.array_equal(rho.view(np.uint32), np.asarray(rho_check, F32).view(np.uint32))
+    nbr = SO.knn(pos[:, :3], p.smoothingRadius)
+    f = SO.features(pos, vel, rho, grad, listed, p.smoothingRadius, p.restDensity, nbr=nbr)
+    return SO.bf16_bits(f), nbr
+
+
 def test_features_match_the_oracle_bit_for_bit(sph, oracle_lib):
     p, pos, vel, s = _small_scene()
     dt_sub = (1.0 / 600.0) / fluid.fluid_substeps(p, 1.0 / 600.0)
     say(f"features n={pos.shape[0]} dt_sub={dt_sub}")
     x = surrogate.features(s, p, dt_sub)
     rho, o = _oracle_density(oracle_lib, p, pos, vel, dt_sub)
-    _, counts, _ = o.grid(p.cellCount)
-    assert counts.max() <= 32, "brute-force kNN equals the grid walk only without capped buckets"
-    want = SO.bf16_bits(SO.features(pos, vel, rho, p.smoothingRadius, p.restDensity))
-    present = SO.from_bits(want)[:, 49:57].sum(axis=1)
+    cells = np.floor(pos[:, :3] / F32(p.smoothingRadius)).astype(np.int64)
+    assert np.bincount(SO._hash(cells, p.cellCount)).max() <= 32, \
+        "brute-force kNN equals the grid walk only without capped buckets"
+    want, nbr = _oracle_features(p, pos, vel, dt_sub, rho_check=rho)
+    present = (nbr >= 0).sum(axis=1)
     say(f"  neighbours present: mean {present.mean():.2f}, min {present.min():.0f}")
     assert present.mean() > 6
     diff = x != want
@@ -206,7 +218,7 @@ def test_training_targets_match_the_oracle(sph, oracle_lib):
         rho = o.particles()[:, 3].copy()
         rho0 = rho if rho0 is None else rho0
         steps.append((o.forces()[:, :3].copy(), rho))
-    feats = SO.bf16_bits(SO.features(pos, vel, rho0, p.smoothingRadius, p.restDensity))
+    feats, _ = _oracle_features(p, pos, vel, dt_sub, rho_check=rho0)
     index = {feats[i].tobytes(): i for i in range(n)}
     assert len(index) == n
     rows = np.array([index[x[r].tobytes()] for r in range(n)])
