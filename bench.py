@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--workload", default="dam_break_1m", choices=["dam_break_1m", "batch64"],
                     help="dam_break_1m: the headline line; batch64: 64 independent scenes in one hipGraph")
     ap.add_argument("--tune", action="store_true", help="batch64: re-run the autotuner instead of its cache")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--surrogate-steps", type=int, default=30)
     return ap.parse_args()
 
 
@@ -148,6 +150,51 @@ def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
     }
 
 
+def surrogate_leg(w, steps: int) -> dict | None:
+    """The same dam break stepped by the learned surrogate (4-substep macro-steps, density
+    check every macro-step, plain fallback above 1 % mean density excess) against the
+    plain path, from the same state; the network is the committed trained one."""
+    from computer_graphics_lab_00_amd import surrogate as S
+    path = S.default_weights()
+    if not path.exists():
+        print(f"[bench] no trained weights at {path}: surrogate leg skipped", file=sys.stderr, flush=True)
+        return None
+    net = S.SurrogateNet()
+    net.load(path)
+    net.set_smoothing(S.DEFAULT_SMOOTHING)
+    net.set_backoff(S.DEFAULT_BACKOFF)
+    r = S.compare(w, net, steps, DT, S.DEFAULT_THRESHOLD, unchecked=True, log=sys.stderr)
+    net.close()
+    out = {
+        "value": r["surrogate_particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "plain_value": r["plain_particle_steps_per_s"],
+        "speedup_vs_plain": r["speedup"],
+        "steps": steps,
+        "macro_steps": r["macro_steps"],
+        "fallbacks": r["fallbacks"],
+        "fallback_rate": r["fallback_rate"],
+        "backed_off_macro_steps": r["backed_off"],
+        "network_share": r["network_share"],
+        "smoothing": S.DEFAULT_SMOOTHING,
+        "max_backoff": S.DEFAULT_BACKOFF,
+        "threshold": r["threshold"],
+        "density_excess_mean": {"plain": r["plain_density_excess_mean"],
+                                "surrogate": r["surrogate_density_excess_mean"]},
+        "density_abs_mean": {"plain": r["plain_density_abs_mean"], "surrogate": r["surrogate_density_abs_mean"]},
+        "density_error_vs_plain": r["density_error_vs_plain"],
+        "energy_drift": {"plain": r["energy_drift_plain"], "surrogate": r["energy_drift_surrogate"]},
+        "unchecked": {"speedup_vs_plain": r.get("unchecked_speedup"),
+                      "density_excess_last": (r.get("unchecked_density_excess") or [None])[-1]},
+        "roofline": None,
+    }
+    if r.get("mlp_tflops"):
+        out["roofline"] = {"bound": "mfma", "kernel": "mlp_fused", "achieved": r["mlp_tflops"], "peak": 2500.0,
+                           "unit": "TFLOP/s", "frac": r["mlp_tflops"] / 2500.0,
+                           "flop_per_unit": S.MLP_FLOP_PER_ROW, "ms_per_macro_step": r["mlp_ms_per_macro_step"]}
+    return out
+
+
 def load_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (counters scaled by a plain copy kernel of known size), or None."""
@@ -189,17 +236,25 @@ def main():
     gpu.synchronize()
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
-    gpu.profile_enable(True)
-    gpu.profile_read()   # reset
+    # The timed region replays the captured step graph, as a caller would run it.
     gpu.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         gpu.step(p, DT)
     gpu.synchronize()
     elapsed = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    # Per-kernel durations: the next steps of the same run, launched eagerly with a HIP
+    # event pair around every kernel on the solver's stream (events cannot sit inside the
+    # replayed graph); kernel durations are the same either way, launch gaps are not.
+    prof_steps = min(args.steps, 20)
+    gpu.profile_enable(True)
+    gpu.profile_read()   # reset
+    for _ in range(prof_steps):
+        gpu.step(p, DT)
+    gpu.synchronize()
     prof = gpu.profile_read()
     gpu.profile_enable(False)
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
     value = n * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
@@ -228,7 +283,13 @@ def main():
                     "achieved": b_alg * n / (substep_ms * 1e-3) / 1e9,
                     "frac": b_alg * n / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels_ms_per_substep": {k: v[0] / max(v[1], 1) for k, v in prof.items()},
+        "profiled_steps": prof_steps,
     }
+
+    sur = None
+    if not args.no_surrogate:
+        print("[bench] surrogate leg", file=sys.stderr, flush=True)
+        sur = surrogate_leg(w, args.surrogate_steps)
 
     cpu = None
     if not args.no_cpu:
@@ -255,6 +316,7 @@ def main():
                    "particle_substeps_per_s": value * substeps},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
     gpu.close()

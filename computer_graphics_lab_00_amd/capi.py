@@ -139,6 +139,7 @@ class SphRolloutStats(C.Structure):
         ("macroSteps", C.c_uint64),
         ("fallbacks", C.c_uint64),
         ("plainSubsteps", C.c_uint64),
+        ("backedOff", C.c_uint64),
         ("maxDeviation", C.c_float),
         ("lastDeviation", C.c_float),
     ]
@@ -229,6 +230,8 @@ def _declare(lib):
         "sph_rollout": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_float,
                                   C.POINTER(SphRolloutStats)]),
         "sph_net_inference_time": (C.c_int, [P, _DP, C.POINTER(C.c_uint64)]),
+        "sph_net_set_smoothing": (C.c_int, [P, C.c_float]),
+        "sph_net_set_backoff": (C.c_int, [P, C.c_uint32]),
         "sph_density_deviation": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, _DP]),
     }
     for name, (res, args) in sigs.items():

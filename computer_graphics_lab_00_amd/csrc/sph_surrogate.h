@@ -21,8 +21,8 @@ constexpr float kLengthScale = 0.01f;    // m: the kernel-radius feature is h / 
 hipError_t launch_features(const sphdev::Launch& L, const float4* pos, const float4* vel, const uint32_t* knn,
                            const float4* aux, float vScale, uint16_t* x, hipStream_t s);
 hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, const float4* vel, const float* y,
-                                  uint32_t ldY, float aScale, uint32_t substeps, float4* posOut, float4* velOut,
-                                  hipStream_t s);
+                                  uint32_t ldY, float aScale, uint32_t substeps, const uint32_t* knn,
+                                  float smoothing, float4* posOut, float4* velOut, hipStream_t s);
 hipError_t launch_deviation(const sphdev::Launch& L, const float4* pos, float2* partial, hipStream_t s);
 hipError_t launch_gather_rows(const uint16_t* x, const uint32_t* ids, const uint32_t* slots, uint32_t count,
                               uint16_t* dst, uint32_t* dstIds, hipStream_t s);

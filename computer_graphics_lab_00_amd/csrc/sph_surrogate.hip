@@ -73,6 +73,9 @@ struct SphNet {
     float2* devPartial = nullptr;
     uint32_t devPartialCap = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    float smoothing = 0.0f;   // XSPH velocity smoothing at the start of a macro-step (0: off)
+    uint32_t maxBackoff = 0;  // after a failed check, macro-steps run plain before the next try
+    uint32_t backoff = 0, skip = 0;
     double mlpMs = 0.0;
     uint64_t mlpRows = 0;
 
@@ -696,6 +699,15 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
     const sphdev::KnnOut ko{net->knn, net->aux};
     for (uint32_t step = 0; step < steps; ++step) {
         for (uint32_t m = 0; m < macros; ++m) {
+            if (net->skip > 0) {   // backing off after a failed check: a plain macro-step
+                net->skip -= 1;
+                for (uint32_t k = 0; k < sphsur::kMacro; ++k)
+                    if (int rc = e.step(dtSub, true)) return rc;
+                st.plainSubsteps += sphsur::kMacro;
+                st.backedOff += 1;
+                prepared = false;
+                continue;
+            }
             if (!prepared) {
                 if (int rc = e.prepare(dtSub, &ko)) return rc;
             }
@@ -707,7 +719,8 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             SPH_HIP(hipEventRecord(net->ev[1], s));
             if (int rc = e.backup_state()) return rc;
             SPH_HIP(sphsur::launch_integrate_macro(L, e.sorted_pos(), e.sorted_vel(), net->iy, 4, sphsur::kAccelScale,
-                                                   sphsur::kMacro, e.state_pos(), e.state_vel(), s));
+                                                   sphsur::kMacro, net->knn, net->smoothing, e.state_pos(),
+                                                   e.state_vel(), s));
             e.commit_sorted();
             if (int rc = e.prepare(dtSub, &ko)) return rc;
             double dev[2];
@@ -728,8 +741,13 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
                 st.fallbacks += 1;
                 st.plainSubsteps += sphsur::kMacro;
                 prepared = false;
+                if (net->maxBackoff > 0) {
+                    net->backoff = net->backoff ? std::min(2 * net->backoff, net->maxBackoff) : 1u;
+                    net->skip = net->backoff;
+                }
             } else {
                 prepared = true;
+                net->backoff = 0;
             }
         }
         for (uint32_t k = 0; k < rest; ++k) {
@@ -738,6 +756,25 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             prepared = false;
         }
     }
+    return SPH_OK;
+}
+
+// After a failed check the next 1, 2, 4, ... (at most maxBackoff) macro-steps run plain
+// before the network is tried again; a passed check resets the sequence. 0 disables it.
+extern "C" int sph_net_set_backoff(SphNet* n, uint32_t maxBackoff) {
+    if (!n) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_net_set_backoff: null net");
+    n->maxBackoff = maxBackoff;
+    n->backoff = 0;
+    n->skip = 0;
+    return SPH_OK;
+}
+
+// XSPH velocity smoothing toward the 8 nearest neighbours' mean velocity at the start of
+// each macro-step, 0 (the default) to 1.
+extern "C" int sph_net_set_smoothing(SphNet* n, float smoothing) {
+    if (!n || !(smoothing >= 0.0f && smoothing <= 1.0f))
+        return fail(SPH_ERR_INVALID_ARGUMENT, "sph_net_set_smoothing: smoothing must be in [0, 1]");
+    n->smoothing = smoothing;
     return SPH_OK;
 }
 

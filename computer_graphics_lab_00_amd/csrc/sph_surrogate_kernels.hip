@@ -118,7 +118,8 @@ __device__ void field_accel(const SphForceField& fd, const float p[3], float out
 __global__ void __launch_bounds__(kBlock)
     k_integrate_macro(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
                       const float* __restrict__ y, uint32_t ldY, float aScale, uint32_t substeps,
-                      float4* __restrict__ posOut, float4* __restrict__ velOut) {
+                      const uint32_t* __restrict__ knn, float smoothing, float4* __restrict__ posOut,
+                      float4* __restrict__ velOut) {
     const Where w = locate(L);
     if (!w.valid) return;
     const StepConsts k = w.d->k;
@@ -129,6 +130,28 @@ __global__ void __launch_bounds__(kBlock)
     const BoundaryConsts& bnd = L.bounds[L.blockScene[sphdev::xcd_block(blockIdx.x, gridDim.x)]];
     float pp[3] = {p0.x, p0.y, p0.z};
     float vv[3] = {v0.x, v0.y, v0.z};
+    if (smoothing > 0.0f) {
+        // XSPH-style smoothing: move the velocity toward the mean of the 8 nearest
+        // neighbours' (Monaghan 1989), which damps the particle-scale velocity noise
+        // that prediction errors inject.
+        float m[3] = {0.0f, 0.0f, 0.0f};
+        float cnt = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t u = knn[(size_t)q * L.total + t];
+            if (u != 0xFFFFFFFFu) {
+                const float4 vu = vel[u];
+                m[0] += vu.x;
+                m[1] += vu.y;
+                m[2] += vu.z;
+                cnt += 1.0f;
+            }
+        }
+        if (cnt > 0.0f) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) vv[a] += smoothing * (m[a] / cnt - vv[a]);
+        }
+    }
     for (uint32_t s = 0; s < substeps; ++s) {
         float fa[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t fi = 0; fi < k.fieldCount; ++fi) {
@@ -271,9 +294,11 @@ hipError_t launch_features(const Launch& L, const float4* pos, const float4* vel
 }
 
 hipError_t launch_integrate_macro(const Launch& L, const float4* pos, const float4* vel, const float* y, uint32_t ldY,
-                                  float aScale, uint32_t substeps, float4* posOut, float4* velOut, hipStream_t s) {
+                                  float aScale, uint32_t substeps, const uint32_t* knn, float smoothing,
+                                  float4* posOut, float4* velOut, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_integrate_macro<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, y, ldY, aScale, substeps, posOut, velOut);
+    k_integrate_macro<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, y, ldY, aScale, substeps, knn, smoothing, posOut,
+                                                  velOut);
     return hipGetLastError();
 }
 

@@ -30,8 +30,13 @@ from .fluid import FluidSolver, Workload, cell_count_for, fluid_substeps
 
 FEATURES = 64
 OUTPUTS = 3
+# MFMA work of one forward pass per particle row, the padded output layer included:
+# 2 (64 x 128 + 128 x 128 + 128 x 128 + 128 x 32)
+MLP_FLOP_PER_ROW = 2 * (64 * 128 + 128 * 128 + 128 * 128 + 128 * 32)
 MACRO = 4
 DEFAULT_THRESHOLD = 0.01
+DEFAULT_SMOOTHING = 0.6   # XSPH smoothing of the measured configuration
+DEFAULT_BACKOFF = 8       # plain macro-steps after repeated failed checks, at most
 _FP = C.POINTER(C.c_float)
 _HP = C.POINTER(C.c_uint16)
 _DP = C.POINTER(C.c_double)
@@ -78,6 +83,7 @@ class SurrogateNet:
         check(self._lib.sph_net_create(device, seed, C.byref(h)))
         self._h = h
         self.param_count = int(self._lib.sph_net_param_count())
+        self.max_backoff = 0
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -140,6 +146,15 @@ class SurrogateNet:
         out = (C.c_double * 3)()
         check(self._lib.sph_net_evaluate(self._h, data._h, begin, end, out))
         return {"network": out[0], "gravity_only": out[1], "hold_first_substep": out[2]}
+
+    def set_smoothing(self, smoothing: float) -> None:
+        """XSPH velocity smoothing at the start of each rollout macro-step (0: off)."""
+        check(self._lib.sph_net_set_smoothing(self._h, smoothing))
+
+    def set_backoff(self, max_backoff: int) -> None:
+        """After a failed check, run 1, 2, 4, ... (at most max_backoff) macro-steps plain."""
+        check(self._lib.sph_net_set_backoff(self._h, max_backoff))
+        self.max_backoff = max_backoff
 
     def inference_time(self) -> tuple[float, int]:
         ms = C.c_double()
@@ -274,11 +289,14 @@ def perturb(solver: FluidSolver, spacing: float, pos_noise: float, vel_noise: fl
 
 
 def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4096, dt: float = 1.0 / 600.0,
-             gap_steps: int = 2, seed: int = 2024, first: int = 0, noise=NOISE, log=None) -> list[tuple[int, int]]:
+             gap_steps: int = 2, seed: int = 2024, first: int = 0, noise=NOISE, net: "SurrogateNet | None" = None,
+             log=None) -> list[tuple[int, int]]:
     """Records `records` macro-steps of each of scenes first..first+scenes-1 into `data`;
-    returns each scene's row range. Between records the scene runs `gap_steps` plain
-    steps so the rows of one scene are not all alike; before record r the state is
-    perturbed by noise[r % len(noise)]."""
+    returns each scene's row range. Between records the scene runs `gap_steps` steps so
+    the rows of one scene are not all alike; before record r the state is perturbed by
+    noise[r % len(noise)]. With `net`, the gap steps are rollout steps of that network
+    with the check off, so the rows come from the states the surrogate itself reaches
+    (data aggregation, Ross et al. 2011) with the plain path's response as target."""
     ranges = []
     solver = None
     cap = 0
@@ -302,8 +320,11 @@ def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4
                 pn, vn = noise[r % len(noise)]
                 perturb(solver, sc.workload.spacing, pn, vn, seed=(seed << 24) + i * 1000 + r)
             data.record(solver, p, dt_sub, min(samples, n), seed=(seed << 20) + i * 1000 + r)
-            for _ in range(gap_steps):
-                solver.step(p, dt)
+            if net is not None:
+                rollout(solver, net, p, dt, gap_steps, float("inf"))
+            else:
+                for _ in range(gap_steps):
+                    solver.step(p, dt)
         ranges.append((begin, len(data)))
         if log is not None and (i - first) % 20 == 19:
             print(f"[surrogate] recorded {i - first + 1}/{scenes} scenes, {len(data)} rows", file=log, flush=True)
@@ -340,12 +361,15 @@ def record_workload(data: TrainingSet, workload: Workload, records: int, samples
 
 def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, records: int = 12,
                   samples: int = 4096, steps: int = 6000, batch: int = 8192, lr: float = 1e-3,
-                  eval_workloads=(), log=sys.stderr) -> dict:
+                  eval_workloads=(), dagger_rounds: int = 0, log=sys.stderr) -> dict:
     """The training recipe: `scenes` trajectories, the last `held_out` kept for evaluation;
-    Adam at lr for `steps` batches, then lr / 10 for a quarter as many. Each workload in
-    `eval_workloads` contributes 8 records of 16384 rows from its own trajectory, used for
-    evaluation only."""
-    data = TrainingSet(scenes * records * samples + len(eval_workloads) * 8 * 16384)
+    Adam at lr for `steps` batches, then lr / 10 for a quarter as many. Each of
+    `dagger_rounds` rounds then adds the same number of scenes recorded along the
+    network's own rollouts (seeds not used before) and trains again on all rows. Each
+    workload in `eval_workloads` contributes 8 records of 16384 rows from its own plain
+    trajectory, used for evaluation only."""
+    per_round = scenes * records * samples
+    data = TrainingSet(per_round * (1 + dagger_rounds) + len(eval_workloads) * 8 * 16384)
     ranges = generate(data, scenes, records, samples, log=log)
     split = ranges[scenes - held_out][0] if held_out else len(data)
     end = len(data)
@@ -354,6 +378,17 @@ def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, reco
     tail = net.train(data, 0, split, batch, max(1, steps // 4), lr * 0.1, seed=11, losses=True)
     report = {"rows": len(data), "train_rows": split, "loss_first": float(losses[0]),
               "loss_last": float(tail[-1])}
+    train_ranges = [(0, split)]
+    for rnd in range(dagger_rounds):
+        b0 = len(data)
+        generate(data, scenes, records, samples, first=scenes * (rnd + 1), net=net, log=log)
+        train_ranges.append((b0, len(data)))
+        # rows of all rounds: train on a window ending at the newest rows and covering them all
+        for k, (r0, r1) in enumerate(train_ranges):
+            net.train(data, r0, r1, batch, steps // (2 * len(train_ranges)), lr * 0.3, seed=100 * rnd + k)
+        tail = net.train(data, b0, len(data), batch, max(1, steps // 8), lr * 0.1, seed=1000 + rnd, losses=True)
+        report[f"dagger_{rnd}_loss_last"] = float(tail[-1])
+        report[f"dagger_{rnd}_rows"] = len(data) - b0
     if held_out:
         report["held_out"] = net.evaluate(data, split, end)
     report["train"] = net.evaluate(data, 0, min(split, 1 << 20))
@@ -375,7 +410,7 @@ def _energy(pos, vel, params: SphParams) -> float:
 
 
 def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 / 600.0,
-            threshold: float = DEFAULT_THRESHOLD, stats_every: int = 1, log=None) -> dict:
+            threshold: float = DEFAULT_THRESHOLD, stats_every: int = 1, unchecked: bool = True, log=None) -> dict:
     """Runs `steps` FluidSolver.step(dt) calls of `workload` with the plain path and with
     the surrogate from the same initial state and returns timing and density/energy
     statistics. The timed runs carry no statistics work; the statistics come from
@@ -392,6 +427,7 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     def run(use_net: bool, timed: bool):
         nonlocal threshold
         solver.load_state(pos0, vel0)
+        net.set_backoff(net.max_backoff)   # every run starts with no back-off pending
         st = SphRolloutStats()
         series = []
         solver.synchronize()
@@ -411,16 +447,17 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     net.inference_time()
     t_sur, st, _ = run(True, True)
     mlp_ms, mlp_rows = net.inference_time()
-    # the same rollout with the check never failing: the cost of the surrogate alone
-    saved = threshold
-    threshold = float("inf")
-    t_free, _, _ = run(True, True)
-    net.inference_time()
-    result["unchecked_ms_per_step"] = 1e3 * t_free / steps
-    result["unchecked_speedup"] = t_plain / t_free
-    _, _, free_series = run(True, False)
-    result["unchecked_density_excess"] = [float(v[0]) for v in free_series]
-    threshold = saved
+    if unchecked:
+        # the same rollout with the check never failing: the cost of the surrogate alone
+        saved = threshold
+        threshold = float("inf")
+        t_free, _, _ = run(True, True)
+        net.inference_time()
+        result["unchecked_ms_per_step"] = 1e3 * t_free / steps
+        result["unchecked_speedup"] = t_plain / t_free
+        _, _, free_series = run(True, False)
+        result["unchecked_density_excess"] = [float(v[0]) for v in free_series]
+        threshold = saved
     result["plain_ms_per_step"] = 1e3 * t_plain / steps
     result["surrogate_ms_per_step"] = 1e3 * t_sur / steps
     result["plain_particle_steps_per_s"] = n * steps / t_plain
@@ -429,11 +466,15 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     result["macro_steps"] = int(st.macroSteps)
     result["fallbacks"] = int(st.fallbacks)
     result["fallback_rate"] = st.fallbacks / max(1, st.macroSteps)
+    result["backed_off"] = int(st.backedOff)
+    result["network_macro_steps"] = int(st.macroSteps - st.fallbacks)
+    result["network_share"] = (st.macroSteps - st.fallbacks) / max(1, st.macroSteps + st.backedOff)
     result["mlp_ms"] = mlp_ms
     result["mlp_rows"] = mlp_rows
     if mlp_ms > 0:
-        flop = 2.0 * sum(nin * nout for nin, nout in ((64, 128), (128, 128), (128, 128), (128, 32))) * mlp_rows
+        flop = MLP_FLOP_PER_ROW * mlp_rows
         result["mlp_tflops"] = flop / (mlp_ms * 1e-3) / 1e12
+        result["mlp_ms_per_macro_step"] = mlp_ms / max(1, st.macroSteps)
     if log is not None:
         print(f"[surrogate] timed: plain {result['plain_ms_per_step']:.3f} ms/step, surrogate "
               f"{result['surrogate_ms_per_step']:.3f} ms/step, fallbacks {st.fallbacks}/{st.macroSteps}",
@@ -474,6 +515,7 @@ def main(argv=None) -> int:
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--out", default=str(default_weights()))
+    t.add_argument("--dagger", type=int, default=0, help="rounds of training on the network's own rollouts")
     t.add_argument("--eval", nargs="*", default=["m1"], choices=("m1", "r4m", "s16k"),
                    help="dam breaks whose own trajectories the network is also evaluated on")
     e = sp.add_parser("compare", help="plain path against the surrogate on a dam break")
@@ -482,6 +524,8 @@ def main(argv=None) -> int:
     e.add_argument("--steps", type=int, default=20)
     e.add_argument("--threshold", type=float, default=DEFAULT_THRESHOLD)
     e.add_argument("--json", default=None)
+    e.add_argument("--smoothing", type=float, default=DEFAULT_SMOOTHING)
+    e.add_argument("--backoff", type=int, default=DEFAULT_BACKOFF)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         import time
@@ -489,7 +533,8 @@ def main(argv=None) -> int:
         t0 = time.perf_counter()
         from . import fluid
         wls = [{"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[w]() for w in a.eval]
-        rep = train_default(net, a.scenes, a.held_out, a.records, a.samples, a.steps, a.batch, a.lr, wls)
+        rep = train_default(net, a.scenes, a.held_out, a.records, a.samples, a.steps, a.batch, a.lr, wls,
+                            a.dagger)
         rep["seconds"] = time.perf_counter() - t0
         Path(a.out).parent.mkdir(parents=True, exist_ok=True)
         net.save(a.out)
@@ -498,6 +543,8 @@ def main(argv=None) -> int:
     from . import fluid
     net = SurrogateNet()
     net.load(a.weights)
+    net.set_smoothing(a.smoothing)
+    net.set_backoff(a.backoff)
     wl = {"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[a.workload]()
     res = compare(wl, net, a.steps, threshold=a.threshold, log=sys.stderr)
     if a.json:

@@ -324,7 +324,8 @@ typedef struct SphDataset SphDataset;
 typedef struct SphRolloutStats {
     uint64_t macroSteps;      /* network macro-steps attempted */
     uint64_t fallbacks;       /* of those, redone with the plain path */
-    uint64_t plainSubsteps;   /* substeps run by the plain path (fallbacks and leftovers) */
+    uint64_t plainSubsteps;   /* substeps run by the plain path (fallbacks, back-off and leftovers) */
+    uint64_t backedOff;       /* macro-steps run plain while backing off after a failed check */
     float maxDeviation;       /* largest mean density excess seen by the check */
     float lastDeviation;
 } SphRolloutStats;
@@ -375,6 +376,14 @@ int sph_features(SphSolver* solver, const SphParams* params, float dtSub, uint16
  * substeps, and every substep of scenes with colliders, run plain. */
 int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* params, float deltaSeconds, uint32_t steps,
                 float threshold, SphRolloutStats* stats);
+/* Optional XSPH smoothing (Monaghan 1989) in sph_rollout: at the start of a macro-step
+ * each velocity moves by `smoothing` (0..1, default 0) toward the mean velocity of its 8
+ * nearest neighbours, damping the particle-scale noise that prediction errors inject. */
+int sph_net_set_smoothing(SphNet* net, float smoothing);
+/* Back-off in sph_rollout: after a failed check the next 1, 2, 4, ... (at most
+ * maxBackoff) macro-steps run plain before the network is tried again; a passed check
+ * resets it. 0 (the default) tries the network on every macro-step. */
+int sph_net_set_backoff(SphNet* net, uint32_t maxBackoff);
 /* Milliseconds spent in the network's forward passes inside sph_rollout since the last
  * call, and the rows they covered; resets both. */
 int sph_net_inference_time(SphNet* net, double* ms, uint64_t* rows);
