@@ -27,6 +27,8 @@ constexpr uint32_t kBlock = 256;          // 4 wave64 per workgroup
 constexpr uint32_t kScanPerThread = 8;    // elements each scan thread owns
 constexpr uint32_t kScanTile = kBlock * kScanPerThread;
 constexpr uint32_t kXcds = 8;             // MI355X: 8 XCDs, each with its own L2
+constexpr uint32_t kPosPad = 8;           // slots of padding after each position array: the
+                                          // density pass reads candidates eight at a time
 
 // Scalars of one substep of one scene, precomputed on the host exactly as the reference
 // computes them (src/physics/fluid_sph.cpp:24-46, :194-195, :264, :271-272) so device and

@@ -264,7 +264,8 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-        if (e == hipSuccess) e = hipMalloc(&pos[i], N * sizeof(float4));
+        if (e == hipSuccess) e = hipMalloc(&pos[i], (N + sphdev::kPosPad) * sizeof(float4));
+        if (e == hipSuccess) e = hipMemset(pos[i] + N, 0, sphdev::kPosPad * sizeof(float4));
         if (e == hipSuccess) e = hipMalloc(&vel[i], N * sizeof(float4));
         if (e == hipSuccess) e = hipMalloc(&ids[i], N * sizeof(uint32_t));
     }

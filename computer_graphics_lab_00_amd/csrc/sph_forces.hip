@@ -80,6 +80,12 @@ __device__ __forceinline__ bool nearer(float r2, uint32_t id, float r2b, uint32_
 // in aux[t] the density gradient of the poly6 estimate,
 // grad rho = -6 cPoly6 m sum (h^2 - r^2)^2 (x_t - x_u), summed in the density's order,
 // with the number of in-range candidates (duplicate visits included, self included).
+// Density pass. One lane per particle walks its 27 buckets; the neighbour-list group is a
+// four-entry shift register and the candidate test is a select, so the per-candidate work
+// is the reference's arithmetic plus a handful of moves. Candidates are read eight at a
+// time from one base address (a typical bucket is one batch, one wait for memory); the
+// last batch of a bucket may read up to seven slots past it (the position arrays carry
+// kPosPad slots of padding) and ignores them.
 template <bool KNN>
 __global__ void __launch_bounds__(kBlock)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
@@ -100,7 +106,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t cap = nbrList ? k.listCap : 0u;
     float density = 0.0f;
     uint32_t listed = 0;
-    uint4 group = make_uint4(0u, 0u, 0u, 0u);
+    uint4 group = make_uint4(0u, 0u, 0u, 0u);   // the last four listed slots, oldest in x
     float kd[8];
     uint32_t ks[8], ki[8];
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
@@ -110,6 +116,51 @@ __global__ void __launch_bounds__(kBlock)
         ks[q] = 0xFFFFFFFFu;
         ki[q] = 0xFFFFFFFFu;
     }
+    // One candidate at slot u, in visiting order; `valid` is false for the padding reads.
+    auto visit = [&](bool valid, uint32_t u, float qx, float qy, float qz) {
+        const float ddx = px - qx, ddy = py - qy, ddz = pz - qz;
+        const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        // poly6 is 0 at r2 >= h2 and density + mass * 0 == density: such candidates leave
+        // the sum as it is.
+        const bool inr = valid && r2 < k.h2;
+        const float d = k.h2 - r2;
+        const float term = k.mass * (((k.cPoly6 * d) * d) * d);
+        density = inr ? density + term : density;
+        const bool put = inr && listed < cap;
+        group.x = put ? group.y : group.x;
+        group.y = put ? group.z : group.y;
+        group.z = put ? group.w : group.z;
+        group.w = put ? u : group.w;
+        if (put && (listed & 3u) == 3u) store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
+        listed += inr ? 1u : 0u;
+        if (KNN && inr) {
+            const float d2 = d * d;
+            gx += d2 * ddx;
+            gy += d2 * ddy;
+            gz += d2 * ddz;
+            if (r2 <= kd[7]) {
+                bool dup = u == t;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) dup |= ks[q] == u;
+                const uint32_t id = dup ? 0xFFFFFFFFu : knnIds[u];
+                if (!dup && nearer(r2, id, kd[7], ki[7])) {
+#pragma unroll
+                    for (int q = 7; q > 0; --q) {
+                        const bool up = nearer(r2, id, kd[q - 1], ki[q - 1]);
+                        const bool here = nearer(r2, id, kd[q], ki[q]);
+                        kd[q] = up ? kd[q - 1] : (here ? r2 : kd[q]);
+                        ks[q] = up ? ks[q - 1] : (here ? u : ks[q]);
+                        ki[q] = up ? ki[q - 1] : (here ? id : ki[q]);
+                    }
+                    if (nearer(r2, id, kd[0], ki[0])) {
+                        kd[0] = r2;
+                        ks[0] = u;
+                        ki[0] = id;
+                    }
+                }
+            }
+        }
+    };
     for (int32_t dz = -1; dz <= 1; ++dz) {
         // The nine buckets of one z layer: all descriptors are fetched before any is used.
         uint32_t bk[9], cn[9], st[9];
@@ -126,64 +177,18 @@ __global__ void __launch_bounds__(kBlock)
             if (cnt == 0) continue;
             const uint32_t lo = st[o];
             if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
-            for (uint32_t j = 0; j < cnt; j += 4) {
-                // Four candidates in flight, then consumed in slot order.
-                float qx[4], qy[4], qz[4];
+            const float4* base = pos + lo;
+            for (uint32_t j = 0; j < cnt; j += 8) {
+                float qx[8], qy[8], qz[8];
 #pragma unroll
-                for (uint32_t e = 0; e < 4; ++e) {
-                    const uint32_t jj = j + e < cnt ? j + e : cnt - 1u;
-                    const float* q = reinterpret_cast<const float*>(pos + lo + jj);
+                for (uint32_t e = 0; e < 8; ++e) {
+                    const float* q = reinterpret_cast<const float*>(base + j + e);
                     qx[e] = q[0];
                     qy[e] = q[1];
                     qz[e] = q[2];
                 }
 #pragma unroll
-                for (uint32_t e = 0; e < 4; ++e) {
-                    const float ddx = px - qx[e], ddy = py - qy[e], ddz = pz - qz[e];
-                    const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                    // poly6 is 0 at r2 >= h2 and density + mass * 0 == density: skip.
-                    if (j + e < cnt && r2 < k.h2) {
-                        const float d = k.h2 - r2;
-                        density += k.mass * (((k.cPoly6 * d) * d) * d);
-                        if (listed < cap) {
-                            const uint32_t u = lo + j + e, lane4 = listed & 3u;
-                            group.x = lane4 == 0u ? u : group.x;
-                            group.y = lane4 == 1u ? u : group.y;
-                            group.z = lane4 == 2u ? u : group.z;
-                            group.w = lane4 == 3u ? u : group.w;
-                            if (lane4 == 3u) store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
-                        }
-                        ++listed;
-                        if (KNN) {
-                            const float d2 = d * d;
-                            gx += d2 * ddx;
-                            gy += d2 * ddy;
-                            gz += d2 * ddz;
-                        }
-                        if (KNN && r2 <= kd[7]) {
-                            const uint32_t u = lo + j + e;
-                            bool dup = u == t;
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) dup |= ks[q] == u;
-                            const uint32_t id = dup ? 0xFFFFFFFFu : knnIds[u];
-                            if (!dup && nearer(r2, id, kd[7], ki[7])) {
-#pragma unroll
-                                for (int q = 7; q > 0; --q) {
-                                    const bool up = nearer(r2, id, kd[q - 1], ki[q - 1]);
-                                    const bool here = nearer(r2, id, kd[q], ki[q]);
-                                    kd[q] = up ? kd[q - 1] : (here ? r2 : kd[q]);
-                                    ks[q] = up ? ks[q - 1] : (here ? u : ks[q]);
-                                    ki[q] = up ? ki[q - 1] : (here ? id : ki[q]);
-                                }
-                                if (nearer(r2, id, kd[0], ki[0])) {
-                                    kd[0] = r2;
-                                    ks[0] = u;
-                                    ki[0] = id;
-                                }
-                            }
-                        }
-                    }
-                }
+                for (uint32_t e = 0; e < 8; ++e) visit(j + e < cnt, lo + j + e, qx[e], qy[e], qz[e]);
             }
         }
     }
@@ -191,8 +196,14 @@ __global__ void __launch_bounds__(kBlock)
     reinterpret_cast<float*>(pos + t)[3] = density;
     reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
     if (nbrCount) {
-        if (listed <= cap && (listed & 3u))
-            store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
+        const uint32_t r = listed & 3u;
+        if (listed <= cap && r) {
+            // the partial group holds its r entries in the last r lanes: move them to the front
+            const uint4 out = r == 1u ? make_uint4(group.w, 0u, 0u, 0u)
+                                      : (r == 2u ? make_uint4(group.z, group.w, 0u, 0u)
+                                                 : make_uint4(group.y, group.z, group.w, 0u));
+            store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, out);
+        }
         nbrCount[t] = listed <= cap ? listed : cap + 1u;
     }
     if (KNN) {
