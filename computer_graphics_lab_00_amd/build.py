@@ -23,6 +23,7 @@ LIB = PKG / "libsph_hip.so"
 
 SOURCES = [
     "sph_host.cpp",
+    "sph_mc_table.cpp",
     "sph_grid.hip",
     "sph_forces.hip",
     "sph_engine.hip",
@@ -31,6 +32,7 @@ SOURCES = [
     "sph_mlp_fused.hip",
     "sph_surrogate_kernels.hip",
     "sph_surrogate.hip",
+    "sph_surface.hip",
 ]
 
 COMMON_FLAGS = [

@@ -134,6 +134,10 @@ class SphSceneCapacity(C.Structure):
     _fields_ = [("maxParticles", C.c_uint32), ("maxCellCount", C.c_uint32)]
 
 
+class SphSurfaceVertex(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("normal", C.c_uint32)]
+
+
 class SphRolloutStats(C.Structure):
     _fields_ = [
         ("macroSteps", C.c_uint64),
@@ -208,6 +212,11 @@ def _declare(lib):
         "sph_debug_step_constants": (C.c_int, [C.POINTER(SphParams), C.c_float, _FP]),
         "sph_debug_copy": (C.c_int, [C.c_uint64]),
         "sph_debug_gemm": (C.c_int, [_HP, _HP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _FP]),
+        "sph_mc_table": (C.c_int, [C.POINTER(C.c_int8)]),
+        "sph_surface_field": (C.c_int, [P, C.POINTER(SphParams), C.c_uint32, _FP]),
+        "sph_march_field": (C.c_int, [_FP, C.c_uint32, _FP, _FP, C.c_float, C.c_void_p, C.c_uint32, _UP]),
+        "sph_surface_mesh": (C.c_int, [P, C.POINTER(SphParams), C.c_uint32, C.c_float, C.c_void_p, C.c_uint32,
+                                       _UP]),
         "sph_net_param_count": (C.c_uint32, []),
         "sph_net_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(P)]),
         "sph_net_destroy": (C.c_int, [P]),

@@ -74,6 +74,11 @@ public:
     // the nearest-neighbour outputs of KnnOut when knn is set). The sorted state is left
     // in the back buffers; nothing is committed.
     int prepare(float dt, const sphdev::KnnOut* knn);
+    // Grid build only (keys, sort, bucket starts) on the current state: the sorted state
+    // and the buckets describe the current positions. Nothing is committed.
+    int build_grid();
+    const uint32_t* bucket_starts() const { return bucketStart; }
+    const uint32_t* bucket_counts() const { return counts; }
     float4* sorted_pos() { return pos[1 - pc]; }
     float4* sorted_vel() { return vel[1 - pc]; }
     uint32_t* sorted_ids() { return ids[1 - ic]; }
@@ -101,7 +106,8 @@ public:
 
 private:
     int ensure_plan(float dt, bool singleSubstep, uint32_t* maxSub);
-    int run_substep(uint32_t j, bool prepareOnly = false, const sphdev::KnnOut* knn = nullptr);
+    int run_substep(uint32_t j, bool prepareOnly = false, const sphdev::KnnOut* knn = nullptr,
+                    bool gridOnly = false);
     int enqueue_step(uint32_t maxSub);
     void invalidate();
     sphdev::Launch launch_for(uint32_t j) const;
@@ -129,6 +135,8 @@ private:
     size_t hostStageBytes = 0;
     std::vector<uint8_t> planKey;
     uint32_t planSub = 0, planBlocks = 0, planBins = 0;
+    float planDt = 1.0f / 600.0f;   // arguments of the last plan, re-used by build_grid
+    bool planSingle = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool hasGrid = false;
     std::vector<TimedLaunch> timed;

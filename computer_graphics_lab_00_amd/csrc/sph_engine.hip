@@ -393,6 +393,8 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     }
     *maxSubOut = maxSub;
     if (maxSub == 0) return SPH_OK;
+    planDt = dt;
+    planSingle = singleSubstep;
     // Key: everything the tables depend on.
     std::vector<uint8_t> key;
     auto put = [&key](const void* p, size_t n) {
@@ -470,7 +472,7 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     return SPH_OK;
 }
 
-int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::KnnOut* knn) {
+int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::KnnOut* knn, bool gridOnly) {
     const sphdev::Launch L = launch_for(j);
     sphdev::GridBuffers g;
     g.pos = pos[pc];
@@ -514,6 +516,7 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::KnnOut* knn)
         Timer t(this, KC_RANK);
         SPH_HIP(sphdev::launch_rank_reorder(L, g, stream));
     }
+    if (gridOnly) return SPH_OK;
     {
         Timer t(this, KC_DENSITY);
         SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, knn,
@@ -574,6 +577,16 @@ int Engine::prepare(float dt, const sphdev::KnnOut* knn) {
     if (int st = ensure_plan(dt, true, &maxSub)) return st;
     if (maxSub == 0) return SPH_OK;
     return run_substep(0, true, knn);
+}
+
+int Engine::build_grid() {
+    if (int st = set_device()) return st;
+    // The grid does not depend on dt: re-validate the last plan (its tables are rebuilt if
+    // counts or params changed since), or make one for a 1/600 s step if there is none.
+    uint32_t maxSub = 0;
+    if (int st = ensure_plan(planDt, planSingle, &maxSub)) return st;
+    if (maxSub == 0) return SPH_OK;
+    return run_substep(0, true, nullptr, true);
 }
 
 int Engine::ensure_force_buffer() {

@@ -1,0 +1,356 @@
+// sph_surface.hip — the fluid's surface on gfx950: the scalar field sampled from the
+// particles on the hash grid (physics::buildFluidField / sampleField,
+// src/physics/marching_cubes.cpp:312-391; shaders/fluid_field.comp) and its iso-surface
+// by marching cubes (physics::marchFluidField, marching_cubes.cpp:393-487;
+// shaders/fluid_marching.comp).
+//
+// The field re-sorts the particles' current positions with the solver's own grid build,
+// so each bucket is one contiguous run in ascending particle index (the reference's
+// FluidGrid order, 32 per bucket), and one lane per sample point walks the 27 buckets
+// around it with the reference's de-duplication, in its order and arithmetic: the field
+// is the reference's bit for bit. Marching runs one lane per cell; vertices go to slots
+// taken with an atomic counter, so their order (not their values) depends on timing, as
+// in the reference's threaded loop.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "sph_engine.h"
+#include "sph_internal.h"
+
+using sphhost::fail;
+using sphhost::hip_fail;
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kMaxResolution = 1024;
+
+__constant__ int8_t c_table[256 * 16];
+
+__device__ __constant__ const int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                                         {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__device__ __constant__ const uint8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                                         {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+struct FieldArgs {
+    float origin[3];
+    float cell[3];
+    float radius;
+    float radiusSq;
+    uint32_t samples;     // resolution + 1 per axis
+    uint32_t resolution;
+    uint32_t cellMask;
+};
+
+__device__ inline uint32_t hash_cell(int32_t cx, int32_t cy, int32_t cz, uint32_t mask) {
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & mask;
+}
+
+// sampleField (marching_cubes.cpp:312-343) at every interior sample; the samples on the
+// grid's faces stay 0 so the surface closes there (:380-383).
+__global__ void __launch_bounds__(kThreads)
+    k_field(const float4* __restrict__ pos, const uint32_t* __restrict__ bucketStart,
+            const uint32_t* __restrict__ counts, FieldArgs a, float* __restrict__ field) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t s = a.samples;
+    if (i >= s * s * s) return;
+    const uint32_t x = i % s, y = (i / s) % s, z = i / (s * s);
+    if (x == 0 || y == 0 || z == 0 || x == a.resolution || y == a.resolution || z == a.resolution) {
+        field[i] = 0.0f;
+        return;
+    }
+    const float px = a.origin[0] + (float)x * a.cell[0];
+    const float py = a.origin[1] + (float)y * a.cell[1];
+    const float pz = a.origin[2] + (float)z * a.cell[2];
+    const int32_t bx = (int32_t)floorf(px / a.radius), by = (int32_t)floorf(py / a.radius),
+                  bz = (int32_t)floorf(pz / a.radius);
+    uint32_t bk[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) bk[o] = hash_cell(bx + o % 3 - 1, by + (o / 3) % 3 - 1, bz + o / 9 - 1, a.cellMask);
+    float total = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        // a bucket two neighbour cells fold into is summed once (:319-326)
+        bool dup = false;
+#pragma unroll
+        for (int q = 0; q < o; ++q) dup |= bk[q] == bk[o];
+        if (dup) continue;
+        uint32_t cnt = counts[bk[o]];
+        if (cnt == 0) continue;
+        if (cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+        const uint32_t lo = bucketStart[bk[o]];
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4 q = pos[lo + k];
+            const float ex = q.x - px, ey = q.y - py, ez = q.z - pz;
+            const float dsq = (ex * ex + ey * ey) + ez * ez;
+            if (dsq >= a.radiusSq) continue;
+            const float d = 1.0f - dsq / a.radiusSq;
+            total += (d * d) * d;
+        }
+    }
+    field[i] = total;
+}
+
+struct MarchArgs {
+    float origin[3];
+    float cellSize[3];
+    float iso;
+    uint32_t resolution;
+    uint32_t capacity;
+};
+
+__device__ inline float field_at(const float* __restrict__ f, uint32_t s, int32_t x, int32_t y, int32_t z) {
+    if (x < 0 || y < 0 || z < 0 || x >= (int32_t)s || y >= (int32_t)s || z >= (int32_t)s) return 0.0f;
+    return f[((size_t)z * s + (size_t)y) * s + (size_t)x];
+}
+
+// asset::packUnitVector (src/asset/vertex_pack.h:40-52): octahedral map to two snorm16.
+__device__ inline uint32_t pack_unit(float nx, float ny, float nz) {
+    const float sum = (fabsf(nx) + fabsf(ny)) + fabsf(nz);
+    if (sum <= 0.0f) return 0u;
+    nx = nx / sum;
+    ny = ny / sum;
+    nz = nz / sum;
+    float u = nx, v = ny;
+    if (nz < 0.0f) {
+        const float sx = nx >= 0.0f ? 1.0f : -1.0f, sy = ny >= 0.0f ? 1.0f : -1.0f;
+        u = (1.0f - fabsf(ny)) * sx;
+        v = (1.0f - fabsf(nx)) * sy;
+    }
+    auto q = [](float c) {
+        c = c < -1.0f ? -1.0f : (1.0f < c ? 1.0f : c);
+        return (uint32_t)(int32_t)roundf(c * 32767.0f) & 0xFFFFu;   // lround: halves away from zero
+    };
+    return q(u) | (q(v) << 16);
+}
+
+// marchFluidField (marching_cubes.cpp:393-487), one lane per cell.
+__global__ void __launch_bounds__(kThreads)
+    k_march(const float* __restrict__ f, MarchArgs m, SphSurfaceVertex* __restrict__ out,
+            uint32_t* __restrict__ counters) {
+    const uint32_t r = m.resolution, s = r + 1;
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= r * r * r) return;
+    const int32_t bx = (int32_t)(i % r), by = (int32_t)((i / r) % r), bz = (int32_t)(i / (r * r));
+    float val[8];
+    uint32_t code = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        val[c] = field_at(f, s, bx + c_corner[c][0], by + c_corner[c][1], bz + c_corner[c][2]);
+        if (val[c] >= m.iso) code |= 1u << c;
+    }
+    const int8_t* entry = c_table + code * 16u;
+    if (entry[0] < 0) return;
+    uint32_t nv = 0;
+    while (nv < 16u && entry[nv] >= 0) ++nv;
+    const uint32_t slot = atomicAdd(&counters[0], nv);
+    if (slot + nv > m.capacity) {
+        atomicMin(&counters[1], slot);   // the first slot that did not fit (:467-472)
+        return;
+    }
+    for (uint32_t v = 0; v < nv; ++v) {
+        const uint32_t e = (uint32_t)entry[v];
+        const uint32_t a = c_edge[e][0], b = c_edge[e][1];
+        const float va = val[a], vb = val[b];
+        float t = fabsf(vb - va) > 1.0e-8f ? (m.iso - va) / (vb - va) : 0.5f;
+        t = t < 0.0f ? 0.0f : (1.0f < t ? 1.0f : t);
+        const int32_t ca[3] = {bx + c_corner[a][0], by + c_corner[a][1], bz + c_corner[a][2]};
+        const int32_t cb[3] = {bx + c_corner[b][0], by + c_corner[b][1], bz + c_corner[b][2]};
+        float p[3], ga[3], gb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float pa = m.origin[k] + (float)ca[k] * m.cellSize[k];
+            const float pb = m.origin[k] + (float)cb[k] * m.cellSize[k];
+            p[k] = pa + (pb - pa) * t;
+        }
+        // central differences over the cell size (:446-455)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int32_t ux = k == 0, uy = k == 1, uz = k == 2;
+            ga[k] = (field_at(f, s, ca[0] + ux, ca[1] + uy, ca[2] + uz) -
+                     field_at(f, s, ca[0] - ux, ca[1] - uy, ca[2] - uz)) / m.cellSize[k];
+            gb[k] = (field_at(f, s, cb[0] + ux, cb[1] + uy, cb[2] + uz) -
+                     field_at(f, s, cb[0] - ux, cb[1] - uy, cb[2] - uz)) / m.cellSize[k];
+        }
+        float n[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) n[k] = -(ga[k] + (gb[k] - ga[k]) * t);   // outward: against the gradient
+        const float dd = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2];
+        if (!(sqrtf(dd) > 1.0e-12f)) {
+            n[0] = 0.0f;
+            n[1] = 1.0f;
+            n[2] = 0.0f;
+        }
+        const float d2 = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2];
+        const float inv = 1.0f / sqrtf(d2);   // glm::normalize: v * inversesqrt(dot(v, v))
+        SphSurfaceVertex o;
+        o.position[0] = p[0];
+        o.position[1] = p[1];
+        o.position[2] = p[2];
+        o.normal = pack_unit(n[0] * inv, n[1] * inv, n[2] * inv);
+        out[slot + v] = o;
+    }
+}
+
+int upload_table() {
+    const int8_t* t = sphhost::mc_table();
+    if (!t) return fail(SPH_ERR_STATE, "marching-cubes table generation failed");
+    SPH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_table), t, 256 * 16));
+    return SPH_OK;
+}
+
+// Field of the solver's current state into the device buffer `field` ((res + 1)^3 floats).
+int field_on_device(sphhost::Engine& e, const SphParams& p, uint32_t resolution, float* field) {
+    const uint32_t s = resolution + 1;
+    const size_t samples = (size_t)s * s * s;
+    if (e.sc[0].n == 0) {
+        SPH_HIP(hipMemsetAsync(field, 0, samples * sizeof(float), e.stream));
+        return SPH_OK;
+    }
+    if (int st = e.build_grid()) return st;
+    FieldArgs a;
+    for (int k = 0; k < 3; ++k) {
+        a.origin[k] = p.containerMin[k];
+        // glm: (containerMax - containerMin) / float(resolution), per component
+        a.cell[k] = (p.containerMax[k] - p.containerMin[k]) / (float)resolution;
+    }
+    a.radius = p.smoothingRadius;
+    a.radiusSq = p.smoothingRadius * p.smoothingRadius;
+    a.samples = s;
+    a.resolution = resolution;
+    a.cellMask = p.cellCount - 1u;
+    k_field<<<(uint32_t)((samples + kThreads - 1) / kThreads), kThreads, 0, e.stream>>>(
+        e.sorted_pos(), e.bucket_starts(), e.bucket_counts(), a, field);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+int march_on_device(const float* field, uint32_t resolution, const float origin[3], const float cellSize[3],
+                    float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written, hipStream_t s) {
+    if (int st = upload_table()) return st;
+    uint32_t* counters = nullptr;
+    SPH_HIP(hipMalloc(&counters, 2 * sizeof(uint32_t)));
+    const uint32_t init[2] = {0u, capacity};
+    hipError_t he = hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, s);
+    MarchArgs m;
+    for (int k = 0; k < 3; ++k) {
+        m.origin[k] = origin[k];
+        m.cellSize[k] = cellSize[k];
+    }
+    m.iso = iso;
+    m.resolution = resolution;
+    m.capacity = capacity;
+    const size_t cells = (size_t)resolution * resolution * resolution;
+    if (he == hipSuccess) {
+        k_march<<<(uint32_t)((cells + kThreads - 1) / kThreads), kThreads, 0, s>>>(field, m, out, counters);
+        he = hipGetLastError();
+    }
+    uint32_t got[2] = {0u, 0u};
+    if (he == hipSuccess) he = hipMemcpyAsync(got, counters, sizeof(got), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    hipFree(counters);
+    if (he != hipSuccess) return hip_fail(he, "marching cubes");
+    *written = got[0] < got[1] ? got[0] : got[1];
+    return SPH_OK;
+}
+
+int check_resolution(uint32_t resolution) {
+    if (resolution < 1 || resolution > kMaxResolution)
+        return fail(SPH_ERR_INVALID_ARGUMENT, "resolution must be in 1..1024");
+    return SPH_OK;
+}
+
+}  // namespace
+
+// physics::buildFluidField for the solver's current particles: (resolution + 1)^3 samples,
+// x fastest, into the host array `field`.
+extern "C" int sph_surface_field(SphSolver* solver, const SphParams* p, uint32_t resolution, float* field) {
+    sphhost::clear_error();
+    if (!solver || !p || !field) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_surface_field: null argument");
+    if (int st = check_resolution(resolution)) return st;
+    sphhost::Engine& e = solver->e;
+    if (int st = e.set_params(0, *p)) return st;
+    if (int st = e.set_device()) return st;
+    const uint32_t s = resolution + 1;
+    const size_t bytes = (size_t)s * s * s * sizeof(float);
+    float* d = nullptr;
+    SPH_HIP(hipMalloc(&d, bytes));
+    int st = field_on_device(e, *p, resolution, d);
+    if (!st) {
+        hipError_t he = hipMemcpyAsync(field, d, bytes, hipMemcpyDeviceToHost, e.stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
+        if (he != hipSuccess) st = hip_fail(he, "sph_surface_field");
+    }
+    hipFree(d);
+    return st;
+}
+
+// physics::marchFluidField over a host field ((resolution + 1)^3 samples, x fastest):
+// at most `capacity` vertices (three per triangle) into `out`, their count in *written.
+extern "C" int sph_march_field(const float* field, uint32_t resolution, const float origin[3],
+                               const float cellSize[3], float iso, SphSurfaceVertex* out, uint32_t capacity,
+                               uint32_t* written) {
+    sphhost::clear_error();
+    if (!field || !origin || !cellSize || !written || (capacity && !out))
+        return fail(SPH_ERR_INVALID_ARGUMENT, "sph_march_field: null argument");
+    if (int st = check_resolution(resolution)) return st;
+    *written = 0;
+    if (capacity < 3) return SPH_OK;
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices <= 0) return fail(SPH_ERR_NO_DEVICE, "no HIP device");
+    const uint32_t s = resolution + 1;
+    const size_t bytes = (size_t)s * s * s * sizeof(float);
+    float* d = nullptr;
+    SphSurfaceVertex* v = nullptr;
+    SPH_HIP(hipMalloc(&d, bytes));
+    hipError_t he = hipMalloc(&v, (size_t)capacity * sizeof(SphSurfaceVertex));
+    int st = SPH_OK;
+    if (he == hipSuccess) he = hipMemcpy(d, field, bytes, hipMemcpyHostToDevice);
+    if (he != hipSuccess) st = hip_fail(he, "sph_march_field");
+    if (!st) st = march_on_device(d, resolution, origin, cellSize, iso, v, capacity, written, nullptr);
+    if (!st && *written) {
+        he = hipMemcpy(out, v, (size_t)*written * sizeof(SphSurfaceVertex), hipMemcpyDeviceToHost);
+        if (he != hipSuccess) st = hip_fail(he, "sph_march_field");
+    }
+    hipFree(d);
+    hipFree(v);
+    return st;
+}
+
+// Field and iso-surface of the solver's current particles in one go, on the device; only
+// the vertices cross to the host. The grid spans the container; iso is dimensionless
+// (the field sums (1 - r^2/h^2)^3 per particle).
+extern "C" int sph_surface_mesh(SphSolver* solver, const SphParams* p, uint32_t resolution, float iso,
+                                SphSurfaceVertex* out, uint32_t capacity, uint32_t* written) {
+    sphhost::clear_error();
+    if (!solver || !p || !written || (capacity && !out))
+        return fail(SPH_ERR_INVALID_ARGUMENT, "sph_surface_mesh: null argument");
+    if (int st = check_resolution(resolution)) return st;
+    *written = 0;
+    if (capacity < 3) return SPH_OK;
+    sphhost::Engine& e = solver->e;
+    if (int st = e.set_params(0, *p)) return st;
+    if (int st = e.set_device()) return st;
+    const uint32_t s = resolution + 1;
+    float* d = nullptr;
+    SphSurfaceVertex* v = nullptr;
+    SPH_HIP(hipMalloc(&d, (size_t)s * s * s * sizeof(float)));
+    hipError_t he = hipMalloc(&v, (size_t)capacity * sizeof(SphSurfaceVertex));
+    int st = he == hipSuccess ? SPH_OK : hip_fail(he, "sph_surface_mesh");
+    if (!st) st = field_on_device(e, *p, resolution, d);
+    float origin[3], cell[3];
+    for (int k = 0; k < 3; ++k) {
+        origin[k] = p->containerMin[k];
+        cell[k] = (p->containerMax[k] - p->containerMin[k]) / (float)resolution;
+    }
+    if (!st) st = march_on_device(d, resolution, origin, cell, iso, v, capacity, written, e.stream);
+    if (!st && *written) {
+        he = hipMemcpy(out, v, (size_t)*written * sizeof(SphSurfaceVertex), hipMemcpyDeviceToHost);
+        if (he != hipSuccess) st = hip_fail(he, "sph_surface_mesh");
+    }
+    hipFree(d);
+    hipFree(v);
+    return st;
+}

@@ -301,6 +301,31 @@ int sph_batch_read_particles(SphBatch* batch, uint32_t scene, float* pos4, float
 int sph_batch_profile_enable(SphBatch* batch, int enable);
 int sph_batch_profile_read(SphBatch* batch, const char** names, double* ms, uint32_t* launches, uint32_t capacity);
 
+/* ---- Fluid surface ----
+ * physics::buildFluidField / physics::marchFluidField (src/physics/marching_cubes.h:44-60,
+ * marching_cubes.cpp:312-487). The field has (resolution + 1)^3 samples spanning the
+ * container, x fastest; each interior sample sums (1 - r^2/h^2)^3 over the particles
+ * within h found through the hash grid (a bucket reached from two cells counted once,
+ * at most 32 per bucket), the samples on the grid's faces are 0. */
+typedef struct SphSurfaceVertex {   /* physics::SurfaceVertex (marching_cubes.h:31-36) */
+    float position[3];
+    uint32_t normal;                /* octahedral snorm16 x 2, asset::packUnitVector */
+} SphSurfaceVertex;
+
+/* The marching-cubes case table, 256 x 16 edge indices, -1 terminated (the reference's
+ * MC_TABLE role, marching_cubes.h:24; generated from the same rule, see sph_mc_table.cpp). */
+int sph_mc_table(int8_t* out);
+/* buildFluidField for the solver's current particles into the host array `field`. */
+int sph_surface_field(SphSolver* solver, const SphParams* params, uint32_t resolution, float* field);
+/* marchFluidField over a host field: at most `capacity` vertices (three per triangle) into
+ * `out`, their count in *written (the reference's capacity rule). Vertex order is not
+ * deterministic; the set of triangles is. */
+int sph_march_field(const float* field, uint32_t resolution, const float origin[3], const float cellSize[3],
+                    float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written);
+/* Both on the device for the solver's current particles (grid over the container). */
+int sph_surface_mesh(SphSolver* solver, const SphParams* params, uint32_t resolution, float iso,
+                     SphSurfaceVertex* out, uint32_t capacity, uint32_t* written);
+
 /* ---- Learned surrogate ----
  * No counterpart in the reference: an addition on top of FluidSolver::step
  * (src/physics/fluid_sph.cpp:151-303). A network predicts each particle's SPH

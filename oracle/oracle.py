@@ -51,6 +51,10 @@ def load():
         "orc_lattice_jitter": (None, [_FP, C.c_uint32, C.c_uint32, C.c_float]),
         "orc_conservation": (None, [_FP, _FP, C.c_uint32, C.POINTER(SphParams), _DP]),
         "orc_hardware_threads": (C.c_uint32, []),
+        "orc_surface_field": (None, [_FP, C.c_uint32, C.POINTER(SphParams), C.c_uint32, _FP]),
+        "orc_pack_unit": (C.c_uint32, [_FP]),
+        "orc_march_field": (C.c_uint32, [_FP, C.c_uint32, _FP, _FP, C.c_float, C.POINTER(C.c_int8), _FP, _UP,
+                                         C.c_uint32]),
     }
     for pre, R in (("orc32", _FP), ("orc64", _DP)):
         sigs.update({
@@ -148,6 +152,35 @@ class OracleSolver:
         self._f("read_grid")(self.h, cell_of.ctypes.data_as(_UP), counts.ctypes.data_as(_UP),
                              slots.ctypes.data_as(_UP))
         return cell_of, counts, slots
+
+
+def surface_field(pos4: np.ndarray, params: SphParams, resolution: int) -> np.ndarray:
+    """buildFluidField (src/physics/marching_cubes.cpp:358-391) on the CPU."""
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    s = resolution + 1
+    out = np.empty(s * s * s, np.float32)
+    load().orc_surface_field(pos4.ctypes.data_as(_FP), pos4.shape[0], C.byref(params), resolution,
+                             out.ctypes.data_as(_FP))
+    return out
+
+
+def march_field(field: np.ndarray, resolution: int, origin, cell, iso: float, table: np.ndarray,
+                capacity: int = 1 << 20):
+    """marchFluidField (marching_cubes.cpp:393-487), serial: (positions n x 3, packed normals n)."""
+    field = np.ascontiguousarray(field, np.float32)
+    o = (C.c_float * 3)(*origin)
+    c = (C.c_float * 3)(*cell)
+    t = np.ascontiguousarray(table, np.int8)
+    pos = np.empty((capacity, 3), np.float32)
+    nrm = np.empty(capacity, np.uint32)
+    n = load().orc_march_field(field.ctypes.data_as(_FP), resolution, o, c, iso, t.ctypes.data_as(C.POINTER(C.c_int8)),
+                               pos.ctypes.data_as(_FP), nrm.ctypes.data_as(_UP), capacity)
+    return pos[:n].copy(), nrm[:n].copy()
+
+
+def pack_unit(n) -> int:
+    v = (C.c_float * 3)(*n)
+    return int(load().orc_pack_unit(v))
 
 
 def derive_params(settings: SphSettings, world, n: int, cell_count: int, colliders=(), fields=()) -> SphParams:
