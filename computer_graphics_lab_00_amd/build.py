@@ -33,6 +33,7 @@ SOURCES = [
     "sph_surrogate_kernels.hip",
     "sph_surrogate.hip",
     "sph_surface.hip",
+    "sph_output.hip",
 ]
 
 COMMON_FLAGS = [

@@ -134,6 +134,25 @@ class SphSceneCapacity(C.Structure):
     _fields_ = [("maxParticles", C.c_uint32), ("maxCellCount", C.c_uint32)]
 
 
+class SphGpuInstance(C.Structure):
+    _fields_ = [("model", C.c_float * 16), ("previousModel", C.c_float * 16), ("normalMatrix", C.c_float * 16),
+                ("meshIndex", C.c_uint32), ("bucket", C.c_uint32), ("bucketBase", C.c_uint32),
+                ("jointOffset", C.c_uint32), ("skinnedVertexOffset", C.c_uint32),
+                ("previousSkinnedVertexOffset", C.c_uint32), ("skinnedMeshletOffset", C.c_uint32),
+                ("visibilityBase", C.c_uint32)]
+
+
+class SphTlasInstance(C.Structure):
+    _fields_ = [("transform", (C.c_float * 4) * 3), ("customIndexMask", C.c_uint32), ("sbtOffsetFlags", C.c_uint32),
+                ("accelerationStructureReference", C.c_uint64)]
+
+
+class SphInstanceConfig(C.Structure):
+    _fields_ = [("instanceBase", C.c_uint32), ("tlasBase", C.c_uint32), ("sphereMesh", C.c_uint32),
+                ("fluidIndex", C.c_uint32), ("sphereBlas", C.c_uint64), ("surfaceBlas", C.c_uint64),
+                ("resetHistory", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class SphSurfaceVertex(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("normal", C.c_uint32)]
 
@@ -213,6 +232,8 @@ def _declare(lib):
         "sph_debug_copy": (C.c_int, [C.c_uint64]),
         "sph_debug_gemm": (C.c_int, [_HP, _HP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _FP]),
         "sph_mc_table": (C.c_int, [C.POINTER(C.c_int8)]),
+        "sph_write_instances": (C.c_int, [P, C.POINTER(SphParams), C.POINTER(SphInstanceConfig), C.c_void_p,
+                                          C.c_void_p]),
         "sph_surface_field": (C.c_int, [P, C.POINTER(SphParams), C.c_uint32, _FP]),
         "sph_march_field": (C.c_int, [_FP, C.c_uint32, _FP, _FP, C.c_float, C.c_void_p, C.c_uint32, _UP]),
         "sph_surface_mesh": (C.c_int, [P, C.POINTER(SphParams), C.c_uint32, C.c_float, C.c_void_p, C.c_uint32,

@@ -86,6 +86,7 @@ public:
     float4* state_vel() { return vel[pc]; }
     uint32_t* state_ids() { return ids[ic]; }
     float4* force_buffer() { return force; }
+    const float4* previous_pos() const { return prev; }   // the kept rendered positions, by index
     // After a kernel wrote the state buffers from the prepared (sorted) ones.
     void commit_sorted() {
         ic = 1 - ic;

@@ -301,6 +301,50 @@ int sph_batch_read_particles(SphBatch* batch, uint32_t scene, float* pos4, float
 int sph_batch_profile_enable(SphBatch* batch, int enable);
 int sph_batch_profile_read(SphBatch* batch, const char** names, double* ms, uint32_t* launches, uint32_t capacity);
 
+/* ---- Renderer export ----
+ * gfx::FluidSimulator::writeCpuInstances (src/gfx/fluid.cpp:377-469) and
+ * shaders/fluid_instances.comp: per particle, a draw instance (gfx::GpuInstance,
+ * src/gfx/geometry.h:98-118) with the sphere's model matrix (translate to the position,
+ * scale by the particle radius) and the previous frame's, and optionally a ray-tracing
+ * instance in VkAccelerationStructureInstanceKHR layout. */
+typedef struct SphGpuInstance {
+    float model[16];            /* column-major */
+    float previousModel[16];
+    float normalMatrix[16];
+    uint32_t meshIndex;
+    uint32_t bucket;
+    uint32_t bucketBase;
+    uint32_t jointOffset;
+    uint32_t skinnedVertexOffset;
+    uint32_t previousSkinnedVertexOffset;
+    uint32_t skinnedMeshletOffset;
+    uint32_t visibilityBase;
+} SphGpuInstance;   /* 224 bytes */
+
+typedef struct SphTlasInstance {
+    float transform[3][4];      /* row-major 3 x 4 */
+    uint32_t customIndexMask;   /* instanceCustomIndex (24 bits) | mask << 24 */
+    uint32_t sbtOffsetFlags;    /* instanceShaderBindingTableRecordOffset (24 bits) | flags << 24 */
+    uint64_t accelerationStructureReference;
+} SphTlasInstance;   /* 64 bytes */
+
+typedef struct SphInstanceConfig {
+    uint32_t instanceBase;      /* first slot of the fluid in the instance array */
+    uint32_t tlasBase;          /* first slot in the ray-tracing instance array */
+    uint32_t sphereMesh;        /* mesh index of the particle sphere */
+    uint32_t fluidIndex;        /* fluid number carried by the surface instance */
+    uint64_t sphereBlas;        /* sphere structure address: one RT instance per particle */
+    uint64_t surfaceBlas;       /* non-zero: one RT instance for the surface, the rest inactive */
+    uint32_t resetHistory;      /* previous model = current model (after emit, teleports) */
+    uint32_t pad;
+} SphInstanceConfig;
+
+/* Writes instances[instanceBase + i] for every particle i and, when tlas is not NULL,
+ * tlas[tlasBase + i]; then keeps the rendered positions as the next previous positions
+ * (FluidSolver::keepRendered). */
+int sph_write_instances(SphSolver* solver, const SphParams* params, const SphInstanceConfig* config,
+                        SphGpuInstance* instances, SphTlasInstance* tlas);
+
 /* ---- Fluid surface ----
  * physics::buildFluidField / physics::marchFluidField (src/physics/marching_cubes.h:44-60,
  * marching_cubes.cpp:312-487). The field has (resolution + 1)^3 samples spanning the
