@@ -153,13 +153,17 @@ __device__ __forceinline__ Where locate(const Launch& L) {
     return w;
 }
 
-// Extra outputs of the density pass for the learned surrogate, per slot t:
-//   slots[q * total + t]  the q-th nearest in-range neighbour by (r2, index), q < 8,
-//                         0xFFFFFFFF where fewer exist;
-//   aux[t]                (grad rho (x, y, z) from the poly6 gradient, in-range count).
-struct KnnOut {
-    uint32_t* slots;
-    float4* aux;
+// Neighbourhood sums the density pass also accumulates for the learned surrogate. For
+// every in-range candidate j of particle t (r2 < h2, visiting order, duplicate visits and
+// t itself included) with s = (h2 - r2) / h2, e = (x_t - x_j) / h and w = v_j - v_t:
+//   sums[0 * total + t] = (sum s^2 e,                 count)
+//   sums[1 * total + t] = sum s^3 (e_x e_x, e_y e_y, e_z e_z, e_x e_y)
+//   sums[2 * total + t] = (sum s^3 e_x e_z, sum s^3 e_y e_z, sum s^2 (w . e), sum s)
+//   sums[3 * total + t] = (sum s w,                   0)
+//   sums[4 * total + t] = (sum s^2 (w x e),           0)
+constexpr uint32_t kSumRows = 5;
+struct NeighbourSums {
+    float4* sums;   // kSumRows x total
 };
 
 // launchers (sph_grid.hip); all batched launches use L.blocks workgroups
@@ -179,8 +183,8 @@ hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hip
 
 // launchers (sph_forces.hip)
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, const KnnOut* knn,
-                          const uint32_t* ids, hipStream_t s);
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
+                          const NeighbourSums* sums, hipStream_t s);
 hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
                         const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s);

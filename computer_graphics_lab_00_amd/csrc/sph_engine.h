@@ -71,9 +71,9 @@ public:
 
     // ---- hooks of the learned surrogate ----
     // Grid build + density pass of one substep of length dt on the current state (with
-    // the nearest-neighbour outputs of KnnOut when knn is set). The sorted state is left
-    // in the back buffers; nothing is committed.
-    int prepare(float dt, const sphdev::KnnOut* knn);
+    // the neighbourhood sums when `sums` is set). The sorted state is left in the back
+    // buffers; nothing is committed.
+    int prepare(float dt, const sphdev::NeighbourSums* sums);
     // Grid build only (keys, sort, bucket starts) on the current state: the sorted state
     // and the buckets describe the current positions. Nothing is committed.
     int build_grid();
@@ -107,7 +107,7 @@ public:
 
 private:
     int ensure_plan(float dt, bool singleSubstep, uint32_t* maxSub);
-    int run_substep(uint32_t j, bool prepareOnly = false, const sphdev::KnnOut* knn = nullptr,
+    int run_substep(uint32_t j, bool prepareOnly = false, const sphdev::NeighbourSums* sums = nullptr,
                     bool gridOnly = false);
     int enqueue_step(uint32_t maxSub);
     void invalidate();

@@ -472,7 +472,7 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     return SPH_OK;
 }
 
-int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::KnnOut* knn, bool gridOnly) {
+int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSums* sums, bool gridOnly) {
     const sphdev::Launch L = launch_for(j);
     sphdev::GridBuffers g;
     g.pos = pos[pc];
@@ -519,8 +519,8 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::KnnOut* knn,
     if (gridOnly) return SPH_OK;
     {
         Timer t(this, KC_DENSITY);
-        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, knn,
-                                       g.idsSorted, stream));
+        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, sums,
+                                       stream));
     }
     if (prepareOnly) return SPH_OK;
     {
@@ -571,12 +571,12 @@ int Engine::step(float dt, bool singleSubstep) {
     return SPH_OK;
 }
 
-int Engine::prepare(float dt, const sphdev::KnnOut* knn) {
+int Engine::prepare(float dt, const sphdev::NeighbourSums* sums) {
     if (int st = set_device()) return st;
     uint32_t maxSub = 0;
     if (int st = ensure_plan(dt, true, &maxSub)) return st;
     if (maxSub == 0) return SPH_OK;
-    return run_substep(0, true, knn);
+    return run_substep(0, true, sums);
 }
 
 int Engine::build_grid() {

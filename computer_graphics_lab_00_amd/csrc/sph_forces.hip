@@ -68,29 +68,19 @@ __device__ inline float eos_pressure(const StepConsts& k, float density) {
 // at groups[g * n + t], so the force pass reads a whole wave's group g in one coalesced
 // access. A particle with more than `listCap` entries stores count = listCap + 1 and the
 // force pass scans its buckets instead.
-// Lexicographic (r2, particle index) order of the nearest-neighbour selection: ties
-// are broken by index, so the choice does not depend on the memory layout.
-__device__ __forceinline__ bool nearer(float r2, uint32_t id, float r2b, uint32_t idb) {
-    return r2 < r2b || (r2 == r2b && id < idb);
-}
-
-// With KNN the pass also keeps, per particle, the 8 nearest in-range candidates other
-// than itself by (r2, index) — a duplicate visit of a bucket cannot enter twice — and
-// writes their slots to knn[k * total + t] (0xFFFFFFFF where fewer than 8 exist), and
-// in aux[t] the density gradient of the poly6 estimate,
-// grad rho = -6 cPoly6 m sum (h^2 - r^2)^2 (x_t - x_u), summed in the density's order,
-// with the number of in-range candidates (duplicate visits included, self included).
 // Density pass. One lane per particle walks its 27 buckets; the neighbour-list group is a
 // four-entry shift register and the candidate test is a select, so the per-candidate work
 // is the reference's arithmetic plus a handful of moves. Candidates are read eight at a
 // time from one base address (a typical bucket is one batch, one wait for memory); the
 // last batch of a bucket may read up to seven slots past it (the position arrays carry
 // kPosPad slots of padding) and ignores them.
-template <bool KNN>
+// With SUMS the pass also accumulates the neighbourhood sums of NeighbourSums
+// (sph_device.h) over the same candidates, in the same order.
+template <bool SUMS>
 __global__ void __launch_bounds__(kBlock)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
               const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
-              uint32_t* __restrict__ knn, float4* __restrict__ aux, const uint32_t* __restrict__ knnIds) {
+              float4* __restrict__ sums) {
     const Where w = locate(L);
     if (!w.valid || !w.d->active) return;
     const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
@@ -107,14 +97,18 @@ __global__ void __launch_bounds__(kBlock)
     float density = 0.0f;
     uint32_t listed = 0;
     uint4 group = make_uint4(0u, 0u, 0u, 0u);   // the last four listed slots, oldest in x
-    float kd[8];
-    uint32_t ks[8], ki[8];
-    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    // neighbourhood sums (SUMS only)
+    float ax[20];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        kd[q] = INFINITY;
-        ks[q] = 0xFFFFFFFFu;
-        ki[q] = 0xFFFFFFFFu;
+    for (int q = 0; q < 20; ++q) ax[q] = 0.0f;
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f, invH2 = 0.0f, invH = 0.0f;
+    if (SUMS) {
+        const float4 v = vel[t];
+        vx = v.x;
+        vy = v.y;
+        vz = v.z;
+        invH2 = 1.0f / k.h2;
+        invH = 1.0f / k.h;
     }
     // One candidate at slot u, in visiting order; `valid` is false for the padding reads.
     auto visit = [&](bool valid, uint32_t u, float qx, float qy, float qz) {
@@ -133,32 +127,30 @@ __global__ void __launch_bounds__(kBlock)
         group.w = put ? u : group.w;
         if (put && (listed & 3u) == 3u) store_stream(nbrList + (size_t)(listed >> 2) * L.total + t, group);
         listed += inr ? 1u : 0u;
-        if (KNN && inr) {
-            const float d2 = d * d;
-            gx += d2 * ddx;
-            gy += d2 * ddy;
-            gz += d2 * ddz;
-            if (r2 <= kd[7]) {
-                bool dup = u == t;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) dup |= ks[q] == u;
-                const uint32_t id = dup ? 0xFFFFFFFFu : knnIds[u];
-                if (!dup && nearer(r2, id, kd[7], ki[7])) {
-#pragma unroll
-                    for (int q = 7; q > 0; --q) {
-                        const bool up = nearer(r2, id, kd[q - 1], ki[q - 1]);
-                        const bool here = nearer(r2, id, kd[q], ki[q]);
-                        kd[q] = up ? kd[q - 1] : (here ? r2 : kd[q]);
-                        ks[q] = up ? ks[q - 1] : (here ? u : ks[q]);
-                        ki[q] = up ? ki[q - 1] : (here ? id : ki[q]);
-                    }
-                    if (nearer(r2, id, kd[0], ki[0])) {
-                        kd[0] = r2;
-                        ks[0] = u;
-                        ki[0] = id;
-                    }
-                }
-            }
+        if (SUMS && inr) {
+            const float sw = d * invH2;
+            const float ex = ddx * invH, ey = ddy * invH, ez = ddz * invH;
+            const float s2 = sw * sw, s3 = s2 * sw;
+            const float4 vj = vel[u];
+            const float wx = vj.x - vx, wy = vj.y - vy, wz = vj.z - vz;
+            ax[0] += s2 * ex;
+            ax[1] += s2 * ey;
+            ax[2] += s2 * ez;
+            ax[3] += 1.0f;
+            ax[4] += s3 * (ex * ex);
+            ax[5] += s3 * (ey * ey);
+            ax[6] += s3 * (ez * ez);
+            ax[7] += s3 * (ex * ey);
+            ax[8] += s3 * (ex * ez);
+            ax[9] += s3 * (ey * ez);
+            ax[10] += s2 * ((wx * ex + wy * ey) + wz * ez);
+            ax[11] += sw;
+            ax[12] += sw * wx;
+            ax[13] += sw * wy;
+            ax[14] += sw * wz;
+            ax[16] += s2 * (wy * ez - wz * ey);
+            ax[17] += s2 * (wz * ex - wx * ez);
+            ax[18] += s2 * (wx * ey - wy * ex);
         }
     };
     for (int32_t dz = -1; dz <= 1; ++dz) {
@@ -206,11 +198,10 @@ __global__ void __launch_bounds__(kBlock)
         }
         nbrCount[t] = listed <= cap ? listed : cap + 1u;
     }
-    if (KNN) {
+    if (SUMS) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) knn[(size_t)q * L.total + t] = ks[q];
-        const float gscale = (-6.0f * k.cPoly6) * k.mass;
-        aux[t] = make_float4(gx * gscale, gy * gscale, gz * gscale, (float)listed);
+        for (uint32_t r = 0; r < kSumRows; ++r)
+            sums[(size_t)r * L.total + t] = make_float4(ax[4 * r], ax[4 * r + 1], ax[4 * r + 2], ax[4 * r + 3]);
     }
 }
 
@@ -533,15 +524,13 @@ __global__ void __launch_bounds__(kBlock)
 }  // namespace
 
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount, const KnnOut* knn,
-                          const uint32_t* ids, hipStream_t s) {
+                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
+                          const NeighbourSums* sums, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    if (knn) {
-        k_density<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, knn->slots,
-                                                    knn->aux, ids);
+    if (sums) {
+        k_density<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, sums->sums);
     } else {
-        k_density<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, nullptr,
-                                                     nullptr, nullptr);
+        k_density<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, nullptr);
     }
     return hipGetLastError();
 }

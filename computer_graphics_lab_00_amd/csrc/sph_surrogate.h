@@ -8,7 +8,7 @@
 
 namespace sphsur {
 
-constexpr uint32_t kFeatures = 64;   // 8 x 7 neighbour features, density, its gradient, h, count, padding
+constexpr uint32_t kFeatures = 64;   // 20 neighbourhood features, zero padded to one MFMA k-block multiple
 constexpr uint32_t kHidden = 128;
 constexpr uint32_t kOutPadded = 32;  // 3 outputs padded to one MFMA tile
 constexpr uint32_t kOutputs = 3;
@@ -18,10 +18,10 @@ constexpr float kAccelScale = 98.1f;     // m/s^2: targets and outputs are in un
 constexpr float kDensityScale = 100.0f;  // density deviations and gradients enter in percent
 constexpr float kLengthScale = 0.01f;    // m: the kernel-radius feature is h / 1 cm
 
-hipError_t launch_features(const sphdev::Launch& L, const float4* pos, const float4* vel, const uint32_t* knn,
-                           const float4* aux, float vScale, uint16_t* x, hipStream_t s);
+hipError_t launch_features(const sphdev::Launch& L, const float4* pos, const float4* sums, float vScale, uint16_t* x,
+                           hipStream_t s);
 hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, const float4* vel, const float* y,
-                                  uint32_t ldY, float aScale, uint32_t substeps, const uint32_t* knn,
+                                  uint32_t ldY, float aScale, uint32_t substeps, const float4* sums,
                                   float smoothing, float4* posOut, float4* velOut, hipStream_t s);
 hipError_t launch_deviation(const sphdev::Launch& L, const float4* pos, float2* partial, hipStream_t s);
 hipError_t launch_gather_rows(const uint16_t* x, const uint32_t* ids, const uint32_t* slots, uint32_t count,

@@ -68,8 +68,7 @@ struct SphNet {
     uint32_t icap = 0;
     uint16_t *ix = nullptr, *ia = nullptr, *ib = nullptr;
     float* iy = nullptr;
-    uint32_t* knn = nullptr;
-    float4* aux = nullptr;
+    float4* sums = nullptr;   // neighbourhood sums, kSumRows x icap
     float2* devPartial = nullptr;
     uint32_t devPartialCap = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -85,7 +84,7 @@ struct SphNet {
         for (void* p : {(void*)w, (void*)m1, (void*)m2, (void*)grad, (void*)frozen, (void*)wb, (void*)x, (void*)h[0],
                         (void*)h[1], (void*)h[2], (void*)dy, (void*)dh[0], (void*)dh[1], (void*)y, (void*)t,
                         (void*)partial, (void*)rowErr, (void*)order, (void*)ix, (void*)ia, (void*)ib, (void*)iy,
-                        (void*)knn, (void*)aux, (void*)devPartial})
+                        (void*)sums, (void*)devPartial})
             hipFree(p);
         for (hipEvent_t e : ev)
             if (e) hipEventDestroy(e);
@@ -102,14 +101,13 @@ struct SphDataset {
     // recording scratch, sized for the solver it records from
     uint32_t scratchN = 0;
     uint16_t* xAll = nullptr;
-    uint32_t* knn = nullptr;
-    float4* aux = nullptr;
+    float4* sums = nullptr;
     float4* acc = nullptr;
     uint32_t *slots = nullptr, *sampleIds = nullptr;
 
     ~SphDataset() {
         hipSetDevice(device);
-        for (void* p : {(void*)x, (void*)t, (void*)t0, (void*)xAll, (void*)knn, (void*)aux, (void*)acc, (void*)slots,
+        for (void* p : {(void*)x, (void*)t, (void*)t0, (void*)xAll, (void*)sums, (void*)acc, (void*)slots,
                         (void*)sampleIds})
             hipFree(p);
     }
@@ -139,15 +137,14 @@ int net_ensure_batch(SphNet* n, uint32_t rowsPad) {
 
 int net_ensure_inference(SphNet* n, uint32_t rowsPad) {
     if (rowsPad <= n->icap) return SPH_OK;
-    for (void* p : {(void*)n->ix, (void*)n->ia, (void*)n->ib, (void*)n->iy, (void*)n->knn, (void*)n->aux}) hipFree(p);
+    for (void* p : {(void*)n->ix, (void*)n->ia, (void*)n->ib, (void*)n->iy, (void*)n->sums}) hipFree(p);
     const size_t R = rowsPad;
     SPH_HIP(hipMalloc(&n->ix, R * 64 * 2));
     SPH_HIP(hipMemset(n->ix, 0, R * 64 * 2));   // padded rows stay zero
     SPH_HIP(hipMalloc(&n->ia, R * 128 * 2));
     SPH_HIP(hipMalloc(&n->ib, R * 128 * 2));
     SPH_HIP(hipMalloc(&n->iy, R * 32 * 4));
-    SPH_HIP(hipMalloc(&n->knn, R * 8 * 4));
-    SPH_HIP(hipMalloc(&n->aux, R * sizeof(float4)));
+    SPH_HIP(hipMalloc(&n->sums, R * sphdev::kSumRows * sizeof(float4)));
     n->icap = rowsPad;
     return SPH_OK;
 }
@@ -489,23 +486,20 @@ extern "C" int sph_dataset_record(SphDataset* d, SphSolver* solver, const SphPar
     if (int st = e.ensure_force_buffer()) return st;
     const uint32_t N = e.total_slots();
     if (d->scratchN < N) {
-        for (void* q : {(void*)d->xAll, (void*)d->knn, (void*)d->aux, (void*)d->acc, (void*)d->slots,
-                        (void*)d->sampleIds})
+        for (void* q : {(void*)d->xAll, (void*)d->sums, (void*)d->acc, (void*)d->slots, (void*)d->sampleIds})
             hipFree(q);
         SPH_HIP(hipMalloc(&d->xAll, (size_t)N * 64 * 2));
-        SPH_HIP(hipMalloc(&d->knn, (size_t)N * 8 * 4));
-        SPH_HIP(hipMalloc(&d->aux, (size_t)N * 16));
+        SPH_HIP(hipMalloc(&d->sums, (size_t)N * sphdev::kSumRows * sizeof(float4)));
         SPH_HIP(hipMalloc(&d->acc, (size_t)N * 16));
         SPH_HIP(hipMalloc(&d->slots, (size_t)N * 4));
         SPH_HIP(hipMalloc(&d->sampleIds, (size_t)N * 4));
         d->scratchN = N;
     }
     hipStream_t s = e.stream;
-    const sphdev::KnnOut ko{d->knn, d->aux};
-    if (int st = e.prepare(dtSub, &ko)) return st;
+    const sphdev::NeighbourSums ns{d->sums};
+    if (int st = e.prepare(dtSub, &ns)) return st;
     const sphdev::Launch L = e.plan_launch();
-    SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), d->knn, d->aux, sphsur::kVelocityScale,
-                                    d->xAll, s));
+    SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), d->sums, sphsur::kVelocityScale, d->xAll, s));
     // `samples` distinct slots: a seeded partial Fisher-Yates shuffle.
     std::vector<uint32_t> perm(n);
     for (uint32_t i = 0; i < n; ++i) perm[i] = i;
@@ -644,18 +638,15 @@ extern "C" int sph_features(SphSolver* solver, const SphParams* p, float dtSub, 
     if (int st = e.set_params(0, *p)) return st;
     const uint32_t n = e.sc[0].n, N = e.total_slots();
     uint16_t *all = nullptr, *ordered = nullptr;
-    uint32_t* knn = nullptr;
-    float4* aux = nullptr;
+    float4* sums = nullptr;
     SPH_HIP(hipMalloc(&all, (size_t)N * 64 * 2));
     SPH_HIP(hipMalloc(&ordered, (size_t)N * 64 * 2));
-    SPH_HIP(hipMalloc(&knn, (size_t)N * 8 * 4));
-    SPH_HIP(hipMalloc(&aux, (size_t)N * 16));
-    const sphdev::KnnOut ko{knn, aux};
-    int st = e.prepare(dtSub, &ko);
+    SPH_HIP(hipMalloc(&sums, (size_t)N * sphdev::kSumRows * sizeof(float4)));
+    const sphdev::NeighbourSums ns{sums};
+    int st = e.prepare(dtSub, &ns);
     if (!st) {
         const sphdev::Launch L = e.plan_launch();
-        hipError_t he = sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), knn, aux, sphsur::kVelocityScale,
-                                                all, e.stream);
+        hipError_t he = sphsur::launch_features(L, e.sorted_pos(), sums, sphsur::kVelocityScale, all, e.stream);
         if (he == hipSuccess) he = sphsur::launch_unpermute_rows(all, e.sorted_ids(), n, ordered, e.stream);
         if (he == hipSuccess) he = hipMemcpyAsync(x, ordered, (size_t)n * 64 * 2, hipMemcpyDeviceToHost, e.stream);
         if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
@@ -663,14 +654,13 @@ extern "C" int sph_features(SphSolver* solver, const SphParams* p, float dtSub, 
     }
     hipFree(all);
     hipFree(ordered);
-    hipFree(knn);
-    hipFree(aux);
+    hipFree(sums);
     return st;
 }
 
 // FluidSolver::step with the surrogate: each run of 4 substeps becomes one network
-// evaluation (kNN features -> MLP -> 4 substeps with the predicted SPH acceleration),
-// followed by the density pass of the next substep as the check; if the mean density
+// evaluation (neighbourhood-sum features -> MLP -> 4 substeps with the predicted SPH
+// acceleration), followed by the density pass of the next substep as the check; if the mean density
 // excess max(rho / rho0 - 1, 0) exceeds `threshold`, the macro-step is redone with the
 // plain path. Substeps left over (substeps % 4) and scenes with colliders run plain.
 extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, float deltaSeconds, uint32_t steps,
@@ -696,7 +686,7 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
     if (int rc = net_ensure_inference(net, R)) return rc;
     hipStream_t s = e.stream;
     bool prepared = false;
-    const sphdev::KnnOut ko{net->knn, net->aux};
+    const sphdev::NeighbourSums ns{net->sums};
     for (uint32_t step = 0; step < steps; ++step) {
         for (uint32_t m = 0; m < macros; ++m) {
             if (net->skip > 0) {   // backing off after a failed check: a plain macro-step
@@ -709,20 +699,19 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
                 continue;
             }
             if (!prepared) {
-                if (int rc = e.prepare(dtSub, &ko)) return rc;
+                if (int rc = e.prepare(dtSub, &ns)) return rc;
             }
             const sphdev::Launch L = e.plan_launch();
-            SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), e.sorted_vel(), net->knn, net->aux,
-                                            sphsur::kVelocityScale, net->ix, s));
+            SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), net->sums, sphsur::kVelocityScale, net->ix, s));
             SPH_HIP(hipEventRecord(net->ev[0], s));
             SPH_HIP(sphmlp::launch_mlp_fused(net->ix, n, net->wb, net->w, reinterpret_cast<float4*>(net->iy), 0, s));
             SPH_HIP(hipEventRecord(net->ev[1], s));
             if (int rc = e.backup_state()) return rc;
             SPH_HIP(sphsur::launch_integrate_macro(L, e.sorted_pos(), e.sorted_vel(), net->iy, 4, sphsur::kAccelScale,
-                                                   sphsur::kMacro, net->knn, net->smoothing, e.state_pos(),
+                                                   sphsur::kMacro, net->sums, net->smoothing, e.state_pos(),
                                                    e.state_vel(), s));
             e.commit_sorted();
-            if (int rc = e.prepare(dtSub, &ko)) return rc;
+            if (int rc = e.prepare(dtSub, &ns)) return rc;
             double dev[2];
             if (int rc = deviation_of(net, e, dev)) return rc;
             float ms = 0.0f;
@@ -769,7 +758,7 @@ extern "C" int sph_net_set_backoff(SphNet* n, uint32_t maxBackoff) {
     return SPH_OK;
 }
 
-// XSPH velocity smoothing toward the 8 nearest neighbours' mean velocity at the start of
+// XSPH velocity smoothing toward the neighbourhood's s-weighted mean velocity at the start of
 // each macro-step, 0 (the default) to 1.
 extern "C" int sph_net_set_smoothing(SphNet* n, float smoothing) {
     if (!n || !(smoothing >= 0.0f && smoothing <= 1.0f))

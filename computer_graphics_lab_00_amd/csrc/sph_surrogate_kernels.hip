@@ -1,8 +1,8 @@
 // sph_surrogate_kernels.hip — particle-side kernels of the learned surrogate (gfx950).
 //
 // The surrogate replaces the force pass of four substeps with one network evaluation:
-// per particle, 64 features of its 8 nearest neighbours and of its density field (from
-// the density pass of the grid) go through the MLP, the predicted SPH acceleration is held for a macro-step of 4
+// per particle, features summed over its neighbourhood by the density pass go through the
+// MLP, the predicted SPH acceleration is held for a macro-step of 4
 // substeps of symplectic Euler with the reference's speed clamp and walls
 // (src/physics/fluid_sph.cpp:260-285), and the next density pass checks the result.
 // Scenes with colliders are not handed to the surrogate: they run the plain path.
@@ -24,53 +24,53 @@ namespace {
 
 __device__ inline float clampf_ref(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
-// Features of particle t (slot order of the prepared state). Row layout, 64 BF16:
-//   7q .. 7q+6: the q-th nearest neighbour (q < 8): (dx, dy, dz) / h, (dvx, dvy, dvz) /
-//               vScale, its 100 (rho / rho0 - 1) (zeros where absent)
-//   56        : own 100 (rho / rho0 - 1)
-//   57 .. 59  : 100 grad rho * h / rho0 (the poly6 gradient of the density pass)
-// Density deviations are a few percent at most, so they enter in percent to be of the
-// same order as the other features.
-//   60        : h / kLengthScale (one network serves several resolutions: at equal
-//               compression the SPH acceleration scales with 1 / h)
-//   61        : in-range candidates of the density pass / 32
-//   62, 63    : 0
-// Offsets are relative to the particle, so the features are translation invariant.
+// Features of particle t (slot order of the prepared state), from the neighbourhood
+// sums of the density pass (NeighbourSums, sph_device.h). Row layout, 64 BF16:
+//   0       : 100 (rho / rho0 - 1)
+//   1 .. 3  : sum s^2 e            (shape of the density gradient)
+//   4 .. 9  : sum s^3 e e^T        (xx, yy, zz, xy, xz, yz: the neighbourhood's anisotropy)
+//   10 .. 12: sum s w / vScale     (smoothed velocity difference: the viscous pull)
+//   13      : sum s^2 (w . e) / vScale   (compression rate)
+//   14 .. 16: sum s^2 (w x e) / vScale   (rotation)
+//   17      : in-range count / 32
+//   18      : sum s / 8
+//   19      : h / kLengthScale     (one network serves several resolutions)
+//   20 .. 63: 0
+// All are relative to the particle (translation invariant); vectors are in world axes,
+// so the network also sees which way gravity points.
 __global__ void __launch_bounds__(kBlock)
-    k_features(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
-               const uint32_t* __restrict__ knn, const float4* __restrict__ aux, float vScale,
+    k_features(Launch L, const float4* __restrict__ pos, const float4* __restrict__ sums, float vScale,
                uint16_t* __restrict__ x) {
     const Where w = locate(L);
     if (!w.valid) return;
     const StepConsts k = w.d->k;
     const uint32_t t = w.slot;
-    const float4 p = pos[t], v = vel[t], a = aux[t];
-    const float invH = 1.0f / k.h, invV = 1.0f / vScale, invRho = 1.0f / k.restDensity;
-    const float ds = kDensityScale;
+    const float4 r0 = sums[t], r1 = sums[(size_t)L.total + t], r2 = sums[2 * (size_t)L.total + t],
+                 r3 = sums[3 * (size_t)L.total + t], r4 = sums[4 * (size_t)L.total + t];
+    const float invV = 1.0f / vScale;
     float f[kFeatures];
 #pragma unroll
     for (int i = 0; i < kFeatures; ++i) f[i] = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const uint32_t u = knn[(size_t)q * L.total + t];
-        if (u != 0xFFFFFFFFu) {
-            const float4 pu = pos[u], vu = vel[u];
-            f[7 * q + 0] = (pu.x - p.x) * invH;
-            f[7 * q + 1] = (pu.y - p.y) * invH;
-            f[7 * q + 2] = (pu.z - p.z) * invH;
-            f[7 * q + 3] = (vu.x - v.x) * invV;
-            f[7 * q + 4] = (vu.y - v.y) * invV;
-            f[7 * q + 5] = (vu.z - v.z) * invV;
-            f[7 * q + 6] = (pu.w * invRho - 1.0f) * ds;
-        }
-    }
-    f[56] = (p.w * invRho - 1.0f) * ds;
-    const float gs = (k.h * invRho) * ds;
-    f[57] = a.x * gs;
-    f[58] = a.y * gs;
-    f[59] = a.z * gs;
-    f[60] = k.h / kLengthScale;
-    f[61] = a.w * (1.0f / 32.0f);
+    f[0] = (pos[t].w / k.restDensity - 1.0f) * kDensityScale;
+    f[1] = r0.x;
+    f[2] = r0.y;
+    f[3] = r0.z;
+    f[4] = r1.x;
+    f[5] = r1.y;
+    f[6] = r1.z;
+    f[7] = r1.w;
+    f[8] = r2.x;
+    f[9] = r2.y;
+    f[10] = r3.x * invV;
+    f[11] = r3.y * invV;
+    f[12] = r3.z * invV;
+    f[13] = r2.z * invV;
+    f[14] = r4.x * invV;
+    f[15] = r4.y * invV;
+    f[16] = r4.z * invV;
+    f[17] = r0.w * (1.0f / 32.0f);
+    f[18] = r2.w * (1.0f / 8.0f);
+    f[19] = k.h / kLengthScale;
     uint4* row = reinterpret_cast<uint4*>(x + (size_t)t * kFeatures);
 #pragma unroll
     for (int c = 0; c < kFeatures / 8; ++c) {
@@ -118,7 +118,7 @@ __device__ void field_accel(const SphForceField& fd, const float p[3], float out
 __global__ void __launch_bounds__(kBlock)
     k_integrate_macro(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
                       const float* __restrict__ y, uint32_t ldY, float aScale, uint32_t substeps,
-                      const uint32_t* __restrict__ knn, float smoothing, float4* __restrict__ posOut,
+                      const float4* __restrict__ sums, float smoothing, float4* __restrict__ posOut,
                       float4* __restrict__ velOut) {
     const Where w = locate(L);
     if (!w.valid) return;
@@ -131,25 +131,15 @@ __global__ void __launch_bounds__(kBlock)
     float pp[3] = {p0.x, p0.y, p0.z};
     float vv[3] = {v0.x, v0.y, v0.z};
     if (smoothing > 0.0f) {
-        // XSPH-style smoothing: move the velocity toward the mean of the 8 nearest
-        // neighbours' (Monaghan 1989), which damps the particle-scale velocity noise
-        // that prediction errors inject.
-        float m[3] = {0.0f, 0.0f, 0.0f};
-        float cnt = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const uint32_t u = knn[(size_t)q * L.total + t];
-            if (u != 0xFFFFFFFFu) {
-                const float4 vu = vel[u];
-                m[0] += vu.x;
-                m[1] += vu.y;
-                m[2] += vu.z;
-                cnt += 1.0f;
-            }
-        }
-        if (cnt > 0.0f) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) vv[a] += smoothing * (m[a] / cnt - vv[a]);
+        // XSPH-style smoothing (Monaghan 1989): move the velocity toward the kernel-weighted
+        // mean of its neighbours', sum s w / sum s, which damps the particle-scale velocity
+        // noise that prediction errors inject.
+        const float4 u = sums[3 * (size_t)L.total + t];
+        const float wsum = sums[2 * (size_t)L.total + t].w;
+        if (wsum > 0.0f) {
+            vv[0] += smoothing * (u.x / wsum);
+            vv[1] += smoothing * (u.y / wsum);
+            vv[2] += smoothing * (u.z / wsum);
         }
     }
     for (uint32_t s = 0; s < substeps; ++s) {
@@ -286,18 +276,18 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
-hipError_t launch_features(const Launch& L, const float4* pos, const float4* vel, const uint32_t* knn,
-                           const float4* aux, float vScale, uint16_t* x, hipStream_t s) {
+hipError_t launch_features(const Launch& L, const float4* pos, const float4* sums, float vScale, uint16_t* x,
+                           hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_features<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, knn, aux, vScale, x);
+    k_features<<<L.blocks, kBlock, 0, s>>>(L, pos, sums, vScale, x);
     return hipGetLastError();
 }
 
 hipError_t launch_integrate_macro(const Launch& L, const float4* pos, const float4* vel, const float* y, uint32_t ldY,
-                                  float aScale, uint32_t substeps, const uint32_t* knn, float smoothing,
+                                  float aScale, uint32_t substeps, const float4* sums, float smoothing,
                                   float4* posOut, float4* velOut, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_integrate_macro<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, y, ldY, aScale, substeps, knn, smoothing, posOut,
+    k_integrate_macro<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, y, ldY, aScale, substeps, sums, smoothing, posOut,
                                                   velOut);
     return hipGetLastError();
 }

@@ -235,39 +235,50 @@ class TrainingScene:
     workload: Workload
     velocity_seed: int
     warmup_steps: int
+    at_rest: bool = False
 
     def initial_state(self):
         p = self.workload.params()
         pos, _ = self.workload.initial_state(p)
         vel = np.zeros_like(pos)
-        vel[:, :3] = gaussian(self.velocity_seed, pos.shape[0] * 3, 0.5).reshape(-1, 3)
+        if not self.at_rest:
+            vel[:, :3] = gaussian(self.velocity_seed, pos.shape[0] * 3, 0.5).reshape(-1, 3)
         return p, pos, vel
 
 
 def training_scene(index: int, seed: int = 2024) -> TrainingScene:
-    """Scene `index` of the seeded training family: one of four 64k lattices at one of the
-    two resolutions, placed at a random spot of a box 1.2-2x its size (resting on the
-    floor half of the time), with velocities from N(0, 0.5 m/s), recorded after 0-47
-    plain steps so collapsing and splashing states are covered."""
+    """Scene `index` of the seeded training family, at one of the two resolutions. Three in
+    four are one of four 64k lattices placed at a random spot of a box 1.2-2x its size
+    (resting on the floor half of the time) with velocities from N(0, 0.5 m/s); the fourth
+    is a dam break at rest — a 16 x 128 x 32 column in the corner of a box 4-8x its length
+    and depth — so the collapsing, hydrostatically loaded columns the benchmarks run are in
+    the data too. Recorded after 0-47 plain steps."""
     u = (splitmix64(seed * 1000003 + index, 8) >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
-    lattice = LATTICES[index % len(LATTICES)]
     h = RADII[(index // len(LATTICES)) % len(RADII)]
     spacing = h * 0.5
-    extent = [n * spacing for n in lattice]
-    box = [extent[a] * (1.2 + 0.8 * u[a]) for a in range(3)]
     r = spacing * 0.5
-    corner = []
-    for a in range(3):
-        slack = box[a] - extent[a] - 2 * r
-        lo = r + slack * u[3 + a]
-        if a == 1 and u[6] < 0.5:
-            lo = r
-        corner.append(lo)
+    dam = index % 4 == 3
+    if dam:
+        lattice = (16, 128, 32)
+        extent = [n * spacing for n in lattice]
+        box = [extent[0] * (4.0 + 4.0 * u[0]), extent[1] * 1.05, extent[2] * (1.0 + 1.0 * u[2])]
+        corner = [r, r, r]
+    else:
+        lattice = LATTICES[index % len(LATTICES)]
+        extent = [n * spacing for n in lattice]
+        box = [extent[a] * (1.2 + 0.8 * u[a]) for a in range(3)]
+        corner = []
+        for a in range(3):
+            slack = box[a] - extent[a] - 2 * r
+            lo = r + slack * u[3 + a]
+            if a == 1 and u[6] < 0.5:
+                lo = r
+            corner.append(lo)
     count = lattice[0] * lattice[1] * lattice[2]
     wl = Workload(f"train_{index}", lattice, spacing, (0.0, 0.0, 0.0), tuple(box), h,
                   cell_count=max(1024, 1 << int(math.ceil(math.log2(2 * count)))),
                   jitter_seed=(seed + index) & 0xFFFFFFFF, corner=tuple(corner))
-    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * 48))
+    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * 48), at_rest=dam)
 
 
 # Position noise (fraction of the lattice spacing) and velocity noise (m/s) added before

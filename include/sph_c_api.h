@@ -373,20 +373,20 @@ int sph_surface_mesh(SphSolver* solver, const SphParams* params, uint32_t resolu
 /* ---- Learned surrogate ----
  * No counterpart in the reference: an addition on top of FluidSolver::step
  * (src/physics/fluid_sph.cpp:151-303). A network predicts each particle's SPH
- * acceleration (pressure + viscosity force / density) from its 8 nearest neighbours and
+ * acceleration (pressure + viscosity force / density) from sums over its neighbourhood and
  * holds it for 4 substeps; the density pass of the next substep checks the result and
  * the plain path redoes any macro-step whose mean density excess is above a threshold.
  *
  * Network: 64 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 operands with FP32 accumulation,
  * FP32 master weights, Adam. Parameters are W[out][in] then b[out] per layer (the last
  * layer padded to 32 outputs, rows 3..31 held at zero); sph_net_param_count() floats.
- * Feature row (64 BF16, bits as uint16): for the q-th nearest in-range neighbour
- * (q < 8, by distance then index) at 7q..7q+6: (dx, dy, dz) / h, (dvx, dvy, dvz) /
- * (1 m/s) and its 100 (rho / rho0 - 1) (zeros where absent); at 56 the particle's own
- * 100 (rho / rho0 - 1); at 57..59 100 grad rho * h / rho0, the gradient of the poly6
- * density estimate; at 60 h / (1 cm); at 61 the density pass's in-range candidate
- * count / 32; zero at 62, 63. Outputs and targets are accelerations in units of
- * 98.1 m/s^2 (10 g). */
+ * Feature row (64 BF16, bits as uint16), from the density pass's own candidates (r2 < h2,
+ * in visiting order, duplicate visits and the particle itself included), with
+ * s = (h2 - r2) / h2, e = (x_t - x_j) / h and w = (v_j - v_t) / (1 m/s): at 0 the
+ * particle's 100 (rho / rho0 - 1); 1..3 sum s^2 e; 4..9 sum s^3 (ex ex, ey ey, ez ez,
+ * ex ey, ex ez, ey ez); 10..12 sum s w; 13 sum s^2 (w . e); 14..16 sum s^2 (w x e);
+ * 17 the candidate count / 32; 18 sum s / 8; 19 h / (1 cm); zero at 20..63. Outputs
+ * and targets are accelerations in units of 98.1 m/s^2 (10 g). */
 typedef struct SphNet SphNet;
 typedef struct SphDataset SphDataset;
 
@@ -446,8 +446,9 @@ int sph_features(SphSolver* solver, const SphParams* params, float dtSub, uint16
 int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* params, float deltaSeconds, uint32_t steps,
                 float threshold, SphRolloutStats* stats);
 /* Optional XSPH smoothing (Monaghan 1989) in sph_rollout: at the start of a macro-step
- * each velocity moves by `smoothing` (0..1, default 0) toward the mean velocity of its 8
- * nearest neighbours, damping the particle-scale noise that prediction errors inject. */
+ * each velocity moves by `smoothing` (0..1, default 0) toward the s-weighted mean velocity
+ * of its neighbourhood (sum s v_j / sum s), damping the particle-scale noise that
+ * prediction errors inject. */
 int sph_net_set_smoothing(SphNet* net, float smoothing);
 /* Back-off in sph_rollout: after a failed check the next 1, 2, 4, ... (at most
  * maxBackoff) macro-steps run plain before the network is tried again; a passed check

@@ -7,16 +7,14 @@ restatement of this project's own definitions, pinned as follows:
 
   bf16()            round-to-nearest-even FP32 -> BF16; pinned against torch's
                     independent conversion (tests/test_surrogate_oracle.py).
-  knn()             brute-force 8 nearest in-range neighbours by (r2, index), FP32
-                    arithmetic in the order the density pass uses; pinned by invariants
-                    (sorted distances, in range, no self) and by the kd-tree of scipy.
   walk()            the density pass's neighbour walk (27 buckets in dz, dy, dx order,
                     members by index, 32 per bucket, duplicate visits kept —
-                    src/physics/fluid_sph.cpp:194-223) with the surrogate's extra sums:
-                    the poly6 density gradient and the in-range count. Pinned by its
-                    density equalling the FP32 SPH oracle's bit for bit and its gradient
-                    equalling central differences of the FP64 density estimate.
-  features()        the feature row from the walk and knn outputs.
+                    src/physics/fluid_sph.cpp:194-223) with the surrogate's neighbourhood
+                    sums accumulated in that order. Pinned by its density equalling the FP32
+                    SPH oracle's bit for bit, by its sums equalling FP64 brute-force sums over
+                    the particles within h (where no bucket is capped or folded), and by
+                    closed forms (a symmetric neighbourhood has zero odd moments).
+  features()        the feature row from the walk's sums.
   forward()/gradients(), exact mode (FP64, no rounding): pinned against torch.autograd in
                     float64 and central finite differences.
   forward()/gradients(), mirror mode: the same, with BF16 rounding where the GPU rounds
@@ -140,24 +138,6 @@ def adam(w, m1, m2, g, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     return w - lr * mhat / (np.sqrt(vhat) + eps), m1, m2
 
 
-def knn(pos3, h):
-    """Slots of the 8 nearest neighbours within h of each particle, by (r2, index), with
-    r2 = (dx dx + dy dy) + dz dz in FP32, dx = p_self - p_other; -1 pads. Brute force,
-    so it equals the grid walk's choice when no bucket is over its 32-particle cap."""
-    p = np.asarray(pos3, dtype=F32)
-    n = p.shape[0]
-    h2 = F32(h) * F32(h)
-    out = np.full((n, 8), -1, dtype=np.int64)
-    for i in range(n):
-        d = p[i][None, :] - p
-        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
-        ok = np.nonzero(r2 < h2)[0]
-        ok = ok[ok != i]
-        order = np.lexsort((ok, r2[ok]))[:8]
-        out[i, : len(order)] = ok[order]
-    return out
-
-
 def _hash(cells, cell_count):
     c = cells.astype(np.int64) & 0xFFFFFFFF
     h = ((c[:, 0] * 73856093) & 0xFFFFFFFF) ^ ((c[:, 1] * 19349663) & 0xFFFFFFFF) ^ \
@@ -165,20 +145,23 @@ def _hash(cells, cell_count):
     return h & (cell_count - 1)
 
 
-def walk(pos4, h, cell_count, mass, c_poly6, cap=32):
-    """(density, grad rho (n x 3), in-range count) of the density pass, summed in FP32 in
-    the pass's visiting order. c_poly6 must be the pass's FP32 constant."""
+def walk(pos4, vel4, h, cell_count, mass, c_poly6, cap=32):
+    """(density, sums (n x 20), in-range count) of the density pass: the density and the
+    neighbourhood sums of the surrogate, summed in FP32 in the pass's visiting order.
+    c_poly6 must be the pass's FP32 constant. sums columns: 0-2 sum s^2 e, 3 count,
+    4-9 sum s^3 (xx, yy, zz, xy, xz, yz), 10 sum s^2 (w . e), 11 sum s, 12-14 sum s w,
+    15 0, 16-18 sum s^2 (w x e), 19 0; s = (h2 - r2) / h2 as d * (1 / h2), e = (x_i -
+    x_j) * (1 / h), w = v_j - v_i."""
     p = np.asarray(pos4, dtype=F32)[:, :3]
+    v = np.asarray(vel4, dtype=F32)[:, :3]
     n = p.shape[0]
     hf = F32(h)
     cells = np.floor(p / hf).astype(np.int64)
     buckets = _hash(cells, cell_count)
     order = np.lexsort((np.arange(n), buckets))
     members = {}
-    for b in np.unique(buckets):
-        members[int(b)] = []
     for i in order:
-        lst = members[int(buckets[i])]
+        lst = members.setdefault(int(buckets[i]), [])
         if len(lst) < cap:
             lst.append(int(i))
     offsets = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
@@ -194,52 +177,65 @@ def walk(pos4, h, cell_count, mass, c_poly6, cap=32):
     for i, c in enumerate(cands):
         cand[i, : len(c)] = c
     h2 = hf * hf
+    inv_h2 = F32(1) / h2
+    inv_h = F32(1) / hf
     m = F32(mass)
     c6 = F32(c_poly6)
     rho = np.zeros(n, F32)
-    g = np.zeros((n, 3), F32)
-    listed = np.zeros(n, np.int64)
+    ax = np.zeros((n, 20), F32)
+    one = F32(1)
     for c in range(width):
         u = cand[:, c]
         valid = u >= 0
-        q = p[np.where(valid, u, 0)]
+        uu = np.where(valid, u, 0)
+        q = p[uu]
         d3 = p - q
         r2 = (d3[:, 0] * d3[:, 0] + d3[:, 1] * d3[:, 1]) + d3[:, 2] * d3[:, 2]
         inr = valid & (r2 < h2)
         d = h2 - r2
         rho = np.where(inr, rho + m * (((c6 * d) * d) * d), rho)
-        d2 = d * d
-        g = np.where(inr[:, None], g + d2[:, None] * d3, g)
-        listed += inr
+        sw = d * inv_h2
+        e = d3 * inv_h
+        s2 = sw * sw
+        s3 = s2 * sw
+        w = v[uu] - v
+        terms = np.zeros((n, 20), F32)
+        terms[:, 0:3] = s2[:, None] * e
+        terms[:, 3] = one
+        terms[:, 4] = s3 * (e[:, 0] * e[:, 0])
+        terms[:, 5] = s3 * (e[:, 1] * e[:, 1])
+        terms[:, 6] = s3 * (e[:, 2] * e[:, 2])
+        terms[:, 7] = s3 * (e[:, 0] * e[:, 1])
+        terms[:, 8] = s3 * (e[:, 0] * e[:, 2])
+        terms[:, 9] = s3 * (e[:, 1] * e[:, 2])
+        terms[:, 10] = s2 * ((w[:, 0] * e[:, 0] + w[:, 1] * e[:, 1]) + w[:, 2] * e[:, 2])
+        terms[:, 11] = sw
+        terms[:, 12:15] = sw[:, None] * w
+        terms[:, 16] = s2 * (w[:, 1] * e[:, 2] - w[:, 2] * e[:, 1])
+        terms[:, 17] = s2 * (w[:, 2] * e[:, 0] - w[:, 0] * e[:, 2])
+        terms[:, 18] = s2 * (w[:, 0] * e[:, 1] - w[:, 1] * e[:, 0])
+        live = np.array([k not in (15, 19) for k in range(20)])
+        ax[:, live] = np.where(inr[:, None], ax[:, live] + terms[:, live], ax[:, live])
     rho = np.where(rho < F32(1e-6), F32(1e-6), rho)
-    gscale = (F32(-6.0) * c6) * m
-    return rho, g * gscale, listed
+    return rho, ax
 
 
-def features(pos4, vel4, density, grad, listed, h, rest_density, nbr=None, v_scale=VELOCITY_SCALE):
-    """Feature rows (n x 64, FP32 values before BF16 rounding) of every particle."""
-    pos = np.asarray(pos4, dtype=F32)[:, :3]
-    vel = np.asarray(vel4, dtype=F32)[:, :3]
+def features(density, sums, h, rest_density, v_scale=VELOCITY_SCALE):
+    """Feature rows (n x 64, FP32 values before BF16 rounding) from walk()'s outputs."""
     rho = np.asarray(density, dtype=F32)
-    if nbr is None:
-        nbr = knn(pos, h)
-    n = pos.shape[0]
-    inv_h = F32(1.0) / F32(h)
+    a = np.asarray(sums, dtype=F32)
+    n = rho.shape[0]
     inv_v = F32(1.0) / F32(v_scale)
-    inv_rho = F32(1.0) / F32(rest_density)
     f = np.zeros((n, FEATURES), dtype=F32)
-    for q in range(8):
-        u = nbr[:, q]
-        has = u >= 0
-        uu = np.where(has, u, 0)
-        f[:, 7 * q: 7 * q + 3] = np.where(has[:, None], (pos[uu] - pos) * inv_h, 0)
-        f[:, 7 * q + 3: 7 * q + 6] = np.where(has[:, None], (vel[uu] - vel) * inv_v, 0)
-        f[:, 7 * q + 6] = np.where(has, (rho[uu] * inv_rho - F32(1.0)) * F32(DENSITY_SCALE), 0)
-    f[:, 56] = (rho * inv_rho - F32(1.0)) * F32(DENSITY_SCALE)
-    gs = (F32(h) * inv_rho) * F32(DENSITY_SCALE)
-    f[:, 57:60] = np.asarray(grad, dtype=F32) * gs
-    f[:, 60] = F32(h) / F32(LENGTH_SCALE)
-    f[:, 61] = np.asarray(listed).astype(F32) * F32(1.0 / 32.0)
+    f[:, 0] = (rho / F32(rest_density) - F32(1.0)) * F32(DENSITY_SCALE)
+    f[:, 1:4] = a[:, 0:3]
+    f[:, 4:10] = a[:, 4:10]
+    f[:, 10:13] = a[:, 12:15] * inv_v
+    f[:, 13] = a[:, 10] * inv_v
+    f[:, 14:17] = a[:, 16:19] * inv_v
+    f[:, 17] = a[:, 3] * F32(1.0 / 32.0)
+    f[:, 18] = a[:, 11] * F32(1.0 / 8.0)
+    f[:, 19] = F32(h) / F32(LENGTH_SCALE)
     return f
 
 

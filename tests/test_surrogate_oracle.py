@@ -1,7 +1,8 @@
 """Pins of the surrogate oracle (oracle/surrogate_oracle.py) on the CPU: BF16 rounding
 against torch, the network's backward pass against torch.autograd and finite
 differences, the BF16-mirroring mode against the exact one, Adam against its closed
-form, and the nearest-neighbour choice against scipy's kd-tree."""
+form, and the neighbourhood walk's density and sums against the SPH oracle, brute force
+over all pairs, central differences and the symmetries of a lattice."""
 import numpy as np
 import pytest
 import torch
@@ -171,22 +172,6 @@ def test_adam_closed_form_for_a_constant_gradient():
     np.testing.assert_allclose(w, expect, rtol=1e-10, atol=1e-14)
 
 
-def test_knn_equals_a_kd_tree_query():
-    from scipy.spatial import cKDTree
-    rng = np.random.default_rng(7)
-    pos = rng.uniform(0, 1, (600, 3)).astype(F32)
-    h = 0.12
-    nbr = S.knn(pos, h)
-    tree = cKDTree(pos.astype(np.float64))
-    d, j = tree.query(pos.astype(np.float64), k=9, distance_upper_bound=h)
-    for i in range(len(pos)):
-        want = [int(u) for u, dd in zip(j[i], d[i]) if np.isfinite(dd) and u != i and dd < h][:8]
-        got = [int(u) for u in nbr[i] if u >= 0]
-        assert got == want, i
-        r = np.linalg.norm(pos[got] - pos[i], axis=1)
-        assert np.all(np.diff(r) >= 0) and np.all(r < h)
-
-
 def _step_constants(p, dt):
     import ctypes as C
     from computer_graphics_lab_00_amd import capi
@@ -207,37 +192,75 @@ def _small_lattice(lattice=(12, 12, 6), cell_count=4096, corner=(0.05, 0.03, 0.0
 @pytest.mark.parametrize("cell_count", [4096, 16])
 def test_walk_density_equals_the_sph_oracle(oracle_lib, sph, cell_count):
     p, pos = _small_lattice(cell_count=cell_count)
+    vel = np.zeros_like(pos)
     dt = 2.0833e-4
     o = oracle_lib.OracleSolver(32)
     o.load_state(pos)
     o.substep(p, dt, 1)
     want = o.particles()[:, 3]
     c = _step_constants(p, dt)
-    rho, grad, listed = S.walk(pos, p.smoothingRadius, p.cellCount, p.particleMass, c[2])
+    rho, sums = S.walk(pos, vel, p.smoothingRadius, p.cellCount, p.particleMass, c[2])
     cells = np.floor(pos[:, :3] / F32(p.smoothingRadius)).astype(np.int64)
     sizes = np.bincount(S._hash(cells, p.cellCount), minlength=p.cellCount)
     if cell_count == 16:
         assert sizes.max() > 32, "the small table must exercise the 32-particle cap"
     assert np.array_equal(rho.view(np.uint32), want.view(np.uint32))
-    assert np.all(np.isfinite(grad))
+    assert np.all(np.isfinite(sums))
     if cell_count == 16:   # particles past the first 32 of their bucket are not candidates
-        assert np.any(listed == 0)
+        assert np.any(sums[:, 3] == 0)
     else:
-        assert np.all(listed >= 1)
+        assert np.all(sums[:, 3] >= 1)
 
 
-def test_walk_gradient_equals_central_differences(sph):
-    # a cloud away from the axes so no two neighbour cells share a bucket
-    rng = np.random.default_rng(11)
+def _cloud(n=400, seed=11, spread=0.015):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), F32)
+    pos[:, :3] = (np.array([1.3, 1.7, 2.1]) + rng.uniform(-spread, spread, (n, 3))).astype(F32)
+    vel = np.zeros((n, 4), F32)
+    vel[:, :3] = rng.normal(0, 0.5, (n, 3)).astype(F32)
+    return pos, vel
+
+
+def test_walk_sums_equal_brute_force():
+    """Away from the axes no two neighbour cells share a bucket and nothing is capped, so
+    the walk visits exactly the particles within h: its FP32 sums equal FP64 brute force."""
     h = 0.0098
-    pos = np.zeros((400, 4), F32)
-    pos[:, :3] = (np.array([1.3, 1.7, 2.1]) + rng.uniform(-0.015, 0.015, (400, 3))).astype(F32)
+    pos, vel = _cloud()
+    c6 = np.float32(315.0 / (64.0 * np.pi * h ** 9))
+    rho, sums = S.walk(pos, vel, h, 1 << 20, 1.1765e-4, c6, cap=1 << 20)
+    x = pos[:, :3].astype(np.float64)
+    v = vel[:, :3].astype(np.float64)
+    h2 = float(F32(h) * F32(h))
+    for i in range(0, 400, 23):
+        d = x[i] - x
+        r2 = np.sum(d * d, axis=1)
+        j = r2 < h2
+        s = (h2 - r2[j]) / h2
+        e = d[j] / float(F32(h))
+        w = v[j] - v[i]
+        want = np.zeros(20)
+        want[0:3] = (s[:, None] ** 2 * e).sum(0)
+        want[3] = j.sum()
+        want[4:10] = [(s ** 3 * e[:, a] * e[:, b]).sum() for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+        want[10] = (s ** 2 * (w * e).sum(1)).sum()
+        want[11] = s.sum()
+        want[12:15] = (s[:, None] * w).sum(0)
+        want[16:19] = (s[:, None] ** 2 * np.cross(w, e)).sum(0)
+        scale = np.abs(want).max() + 1.0
+        assert np.allclose(sums[i], want, rtol=0, atol=2e-5 * scale), i
+
+
+def test_walk_gradient_equals_central_differences():
+    # the first three sums are -grad rho / (6 cPoly6 m h^5): check against differences of
+    # the FP64 density estimate
+    h = 0.0098
+    pos, vel = _cloud()
     mass = 1.1765e-4
     c6 = np.float32(315.0 / (64.0 * np.pi * h ** 9))
-    rho, grad, _ = S.walk(pos, h, 1 << 20, mass, c6, cap=1 << 20)
+    rho, sums = S.walk(pos, vel, h, 1 << 20, mass, c6, cap=1 << 20)
 
-    def estimate(x, skip):
-        d = pos[:, :3].astype(np.float64) - x
+    def estimate(xp, skip):
+        d = pos[:, :3].astype(np.float64) - xp
         r2 = np.sum(d * d, axis=1)
         w = np.where(r2 < h * h, (h * h - r2) ** 3, 0.0)
         w[skip] = 0.0
@@ -245,25 +268,38 @@ def test_walk_gradient_equals_central_differences(sph):
 
     eps = 1e-7
     for i in range(0, 400, 37):
-        x = pos[i, :3].astype(np.float64)
-        fd = np.array([(estimate(x + e, i) - estimate(x - e, i)) / (2 * eps) for e in np.eye(3) * eps])
+        xp = pos[i, :3].astype(np.float64)
+        fd = np.array([(estimate(xp + e, i) - estimate(xp - e, i)) / (2 * eps) for e in np.eye(3) * eps])
+        grad = -6.0 * float(c6) * mass * h ** 5 * sums[i, 0:3].astype(np.float64)
         scale = np.linalg.norm(fd) + 1e-3 * mass * float(c6) * h ** 5
-        assert np.linalg.norm(grad[i] - fd) / scale < 2e-3, i
+        assert np.linalg.norm(grad - fd) / scale < 2e-3, i
+
+
+def test_symmetric_neighbourhood_has_no_odd_moments():
+    h = F32(0.1)
+    g = np.array([-0.03, 0.0, 0.03], F32)
+    pts = np.array([[1.35 + a, 1.75 + b, 2.15 + c, 0] for c in g for b in g for a in g], F32)
+    vel = np.zeros_like(pts)
+    rho, sums = S.walk(pts, vel, h, 1 << 16, 0.001, F32(1.0), cap=64)
+    centre = 13
+    assert np.all(np.abs(sums[centre, 0:3]) < 1e-6)
+    assert np.all(np.abs(sums[centre, 7:10]) < 1e-6)          # off-diagonal moments
+    assert sums[centre, 4] == pytest.approx(sums[centre, 5], rel=1e-5)
+    assert sums[centre, 4] == pytest.approx(sums[centre, 6], rel=1e-5)
+    assert sums[centre, 3] == 27
 
 
 def test_feature_row_layout():
-    pos = np.array([[0, 0, 0, 0], [0.01, 0, 0, 0], [0, 0.02, 0, 0], [1, 1, 1, 0]], dtype=F32)
-    vel = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 0, -2, 0], [0, 0, 0, 0]], dtype=F32)
-    rho = np.array([1010, 990, 1000, 1000], dtype=F32)
-    grad = np.array([[100, 0, -50], [0, 0, 0], [0, 0, 0], [0, 0, 0]], dtype=F32)
-    listed = np.array([3, 3, 3, 1])
-    f = S.features(pos, vel, rho, grad, listed, 0.05, 1000.0)
-    assert f.shape == (4, 64)
-    np.testing.assert_allclose(f[0, 0:7], [0.2, 0, 0, 1, 0, 0, -1.0], rtol=1e-5)
-    np.testing.assert_allclose(f[0, 7:14], [0, 0.4, 0, 0, 0, -2, 0], rtol=1e-5, atol=1e-7)
-    assert not np.any(f[0, 14:56])
-    assert f[0, 56] == (F32(1010) * (F32(1) / F32(1000)) - 1) * F32(100)
-    np.testing.assert_allclose(f[0, 57:60], [0.5, 0, -0.25], rtol=1e-5)
-    assert np.all(f[:, 60] == F32(0.05) / F32(0.01))
-    assert f[0, 61] == F32(3 / 32) and f[3, 61] == F32(1 / 32)
-    assert not np.any(f[3, :56]) and not np.any(f[:, 62:])
+    rho = np.array([1010, 990], dtype=F32)
+    sums = np.zeros((2, 20), F32)
+    sums[0, :] = np.arange(1, 21, dtype=F32)
+    f = S.features(rho, sums, 0.05, 1000.0)
+    assert f.shape == (2, 64)
+    assert f[0, 0] == (F32(1010) / F32(1000) - 1) * F32(100)
+    assert list(f[0, 1:4]) == [1, 2, 3]
+    assert list(f[0, 4:10]) == [5, 6, 7, 8, 9, 10]
+    assert list(f[0, 10:13]) == [13, 14, 15] and f[0, 13] == 11
+    assert list(f[0, 14:17]) == [17, 18, 19]
+    assert f[0, 17] == F32(4 / 32) and f[0, 18] == F32(12 / 8)
+    assert np.all(f[:, 19] == F32(0.05) / F32(0.01))
+    assert not np.any(f[:, 20:]) and not np.any(f[1, 1:19])
