@@ -91,12 +91,15 @@ public:
     void commit_sorted() {
         ic = 1 - ic;
         hasGrid = true;
+        touch();
     }
+    // A value unique in the process that changes whenever the state or the prepared
+    // (sorted) buffers change: equal epochs mean nothing was written in between.
+    uint64_t epoch() const { return epoch_; }
+    void touch();
     sphdev::Launch plan_launch() const { return launch_for(0); }
     uint32_t total_slots() const { return totalN; }
     int ensure_force_buffer();
-    int backup_state();
-    int restore_state();
     uint32_t planned_blocks() const { return planBlocks; }
 
     std::vector<SceneHost> sc;
@@ -113,6 +116,7 @@ private:
     void invalidate();
     sphdev::Launch launch_for(uint32_t j) const;
 
+    uint64_t epoch_ = 0;
     uint32_t totalN = 0, totalC = 0, totalBins = 0, maxBlocks = 0;
     float4* pos[2] = {nullptr, nullptr};
     float4* vel[2] = {nullptr, nullptr};
@@ -126,8 +130,6 @@ private:
     uint32_t* nbrCount = nullptr;
     float4 *prev = nullptr, *force = nullptr, *staging = nullptr;
     uint32_t* stagingU = nullptr;
-    float4 *backupPos = nullptr, *backupVel = nullptr;
-    uint32_t* backupIds = nullptr;
     // plan: per-substep scene tables, boundaries, block -> scene
     sphdev::SceneDesc* dTables = nullptr;
     sphdev::BoundaryConsts* dBounds = nullptr;

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -199,9 +200,6 @@ void Engine::free_all() {
     hipFree(force);
     hipFree(staging);
     hipFree(stagingU);
-    hipFree(backupPos);
-    hipFree(backupVel);
-    hipFree(backupIds);
     hipFree(dTables);
     hipFree(dBounds);
     hipFree(dBlockScene);
@@ -304,6 +302,12 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     return SPH_OK;
 }
 
+namespace {
+std::atomic<uint64_t> g_epoch{0};
+}
+
+void Engine::touch() { epoch_ = ++g_epoch; }
+
 void Engine::invalidate() {
     planKey.clear();
     for (int i = 0; i < 2; ++i) {
@@ -317,6 +321,7 @@ int Engine::set_params(uint32_t scene, const SphParams& p) {
     if (!is_pow2(p.cellCount)) return fail(SPH_ERR_NOT_POWER_OF_TWO, "cellCount must be a power of two");
     if (p.cellCount > sc[scene].maxC) return fail(SPH_ERR_CAPACITY, "cellCount exceeds the scene's maxCellCount");
     if (!(p.smoothingRadius > 0.0f)) return fail(SPH_ERR_INVALID_ARGUMENT, "smoothingRadius must be > 0");
+    if (!sc[scene].hasParams || memcmp(&sc[scene].params, &p, sizeof(SphParams)) != 0) touch();
     sc[scene].params = p;
     sc[scene].hasParams = true;
     return SPH_OK;
@@ -329,6 +334,7 @@ int Engine::emit(uint32_t scene, const SphParams& p, uint32_t count) {
     if (int st = set_device()) return st;
     s.n = count;
     hasGrid = false;
+    touch();
     SPH_HIP(sphdev::launch_emit(p, count, pos[pc] + s.offset, vel[pc] + s.offset, prev + s.offset,
                                 ids[ic] + s.offset, stream));
     return SPH_OK;
@@ -342,6 +348,7 @@ int Engine::load_state(uint32_t scene, const float* pos4, const float* vel4, uin
     if (int st = set_device()) return st;
     s.n = count;
     hasGrid = false;
+    touch();
     if (count == 0) return SPH_OK;
     const size_t bytes = (size_t)count * sizeof(float4);
     SPH_HIP(hipMemcpyAsync(pos[pc] + s.offset, pos4, bytes, hipMemcpyHostToDevice, stream));
@@ -519,8 +526,10 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
     if (gridOnly) return SPH_OK;
     {
         Timer t(this, KC_DENSITY);
-        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, sums,
-                                       stream));
+        // A prepare-only pass is followed by no force pass here: no neighbour list.
+        const bool list = !prepareOnly;
+        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, list ? nbrList : nullptr,
+                                       list ? nbrCount : nullptr, sums, stream));
     }
     if (prepareOnly) return SPH_OK;
     {
@@ -541,6 +550,7 @@ int Engine::enqueue_step(uint32_t maxSub) {
 
 int Engine::step(float dt, bool singleSubstep) {
     if (int st = set_device()) return st;
+    touch();
     uint32_t maxSub = 0;
     if (int st = ensure_plan(dt, singleSubstep, &maxSub)) return st;
     if (maxSub == 0) return SPH_OK;
@@ -573,6 +583,7 @@ int Engine::step(float dt, bool singleSubstep) {
 
 int Engine::prepare(float dt, const sphdev::NeighbourSums* sums) {
     if (int st = set_device()) return st;
+    touch();
     uint32_t maxSub = 0;
     if (int st = ensure_plan(dt, true, &maxSub)) return st;
     if (maxSub == 0) return SPH_OK;
@@ -581,6 +592,7 @@ int Engine::prepare(float dt, const sphdev::NeighbourSums* sums) {
 
 int Engine::build_grid() {
     if (int st = set_device()) return st;
+    touch();
     // The grid does not depend on dt: re-validate the last plan (its tables are rebuilt if
     // counts or params changed since), or make one for a 1/600 s step if there is none.
     uint32_t maxSub = 0;
@@ -594,28 +606,6 @@ int Engine::ensure_force_buffer() {
     if (int st = set_device()) return st;
     SPH_HIP(hipMalloc(&force, (size_t)totalN * sizeof(float4)));
     invalidate();   // captured graphs wrote no forces
-    return SPH_OK;
-}
-
-int Engine::backup_state() {
-    if (int st = set_device()) return st;
-    if (!backupPos) {
-        SPH_HIP(hipMalloc(&backupPos, (size_t)totalN * sizeof(float4)));
-        SPH_HIP(hipMalloc(&backupVel, (size_t)totalN * sizeof(float4)));
-        SPH_HIP(hipMalloc(&backupIds, (size_t)totalN * sizeof(uint32_t)));
-    }
-    SPH_HIP(hipMemcpyAsync(backupPos, pos[pc], (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-    SPH_HIP(hipMemcpyAsync(backupVel, vel[pc], (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-    SPH_HIP(hipMemcpyAsync(backupIds, ids[ic], (size_t)totalN * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-    return SPH_OK;
-}
-
-int Engine::restore_state() {
-    if (!backupPos) return fail(SPH_ERR_STATE, "no backup to restore");
-    if (int st = set_device()) return st;
-    SPH_HIP(hipMemcpyAsync(pos[pc], backupPos, (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-    SPH_HIP(hipMemcpyAsync(vel[pc], backupVel, (size_t)totalN * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-    SPH_HIP(hipMemcpyAsync(ids[ic], backupIds, (size_t)totalN * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
     return SPH_OK;
 }
 

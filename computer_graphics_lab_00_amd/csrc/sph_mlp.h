@@ -60,6 +60,12 @@ hipError_t launch_to_bf16(const float* src, uint32_t count, uint16_t* dst, hipSt
 // mirrors (W[out][in], b[out] per layer). blocks = 0 picks one workgroup per CU.
 hipError_t launch_mlp_fused(const uint16_t* x, uint32_t rows, const uint16_t* wBf16, const float* wF32, float4* y,
                             uint32_t blocks, hipStream_t s);
+// The same network pass with the rows built in the kernel from the prepared state of a
+// single scene at slot offset 0: pos (density in .w) and the neighbourhood sums
+// (kSumRows x total float4), for rows <= total particles.
+hipError_t launch_mlp_fused_sums(const float4* pos, const float4* sums, uint32_t total, uint32_t rows, float rho0,
+                                 float h, float invV, const uint16_t* wBf16, const float* wF32, float4* y,
+                                 uint32_t blocks, hipStream_t s);
 uint32_t mlp_fused_blocks(uint32_t rows);
 
 }  // namespace sphmlp

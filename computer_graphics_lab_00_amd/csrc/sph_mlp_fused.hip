@@ -4,7 +4,10 @@
 // stages the BF16 weights (97 KB, rows padded so LDS reads are conflict free) and the
 // biases in LDS once, and each wave then streams groups of 64 feature rows through the
 // network. HBM traffic is one read of the 128-byte feature row and one 16-byte write of
-// the output per particle; the work is MFMA bound.
+// the output per particle; the work is MFMA bound. The rollout's variant builds the rows
+// itself from the density pass's neighbourhood sums (neighbourhood_features): 96 bytes
+// read per particle and no feature array; its all-zero inputs 20..63 are skipped, which
+// leaves every ReLU output, and so the result, unchanged.
 //
 // Each layer computes the transposed product out^T = W * act^T with
 // v_mfma_f32_32x32x16_bf16: A = W (32 output features x 16 inputs, from LDS), B = act^T
@@ -20,6 +23,7 @@
 #include <stdint.h>
 
 #include "sph_mlp.h"
+#include "sph_surrogate.h"
 
 namespace sphmlp {
 
@@ -69,11 +73,12 @@ __device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias
     }
 }
 
-template <int KB>
+// One layer over the first KB k-blocks of its (LD-strided) staged weights.
+template <int KB, uint32_t LD, int KIN>
 __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                             uint32_t lane, const bf16x8 (&in)[2][KB], bf16x8 (&out)[2][8]) {
+                                             uint32_t lane, const bf16x8 (&in)[2][KIN], bf16x8 (&out)[2][8]) {
     const uint32_t r = lane & 31u, half = lane >> 5;
-    const uint32_t ld = KB == 4 ? kLd0 : kLdH;
+    const uint32_t ld = LD;
     f32x16 acc[2][4];
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt)
@@ -95,9 +100,18 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, con
         for (int mt = 0; mt < 4; ++mt) to_operands(acc[nt][mt], bias + mt * 32, half, out[nt][2 * mt], out[nt][2 * mt + 1]);
 }
 
+struct FusedInput {
+    const uint16_t* x;     // feature rows (rows x 64 BF16), or
+    const float4* pos;     // the prepared state (density in .w) and
+    const float4* sums;    // its neighbourhood sums (kSumRows x total), when SUMS
+    uint32_t total;
+    float rho0, h, invV;
+};
+
+template <bool SUMS>
 __global__ void __launch_bounds__(kThreads)
-    k_mlp_fused(const uint16_t* __restrict__ x, uint32_t rows, const uint16_t* __restrict__ wb,
-                const float* __restrict__ wf, float4* __restrict__ y) {
+    k_mlp_fused(FusedInput in, uint32_t rows, const uint16_t* __restrict__ wb, const float* __restrict__ wf,
+                float4* __restrict__ y) {
     __shared__ __attribute__((aligned(16))) uint16_t sw[kLdsWeights];
     __shared__ float sb[kBiases];
     const uint32_t tid = threadIdx.x;
@@ -119,30 +133,55 @@ __global__ void __launch_bounds__(kThreads)
     if (tid < 32u) sb[384u + tid] = wf[kB3 + tid];
     __syncthreads();
 
+    constexpr int KB0 = SUMS ? 2 : 4;   // k-blocks of layer 0 that can be non-zero
+    static_assert(!SUMS || sphsur::kUsedFeatures <= 32, "the used features must fit two k-blocks");
     const uint32_t lane = tid & 63u, r = lane & 31u, half = lane >> 5;
     const uint32_t stride = gridDim.x * kWaves * 64u;
     for (uint32_t base = (blockIdx.x * kWaves + (tid >> 6)) * 64u; base < rows; base += stride) {
         // The weight fragments are loop invariant; without this the compiler hoists all
         // of them into registers and spills. They are re-read from LDS per row group.
         asm volatile("" ::: "memory");
-        bf16x8 in0[2][4];
+        bf16x8 in0[2][KB0];
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
             const uint32_t row = base + (uint32_t)nt * 32u + r;
+            if (SUMS) {
+                // lane (row, half) holds inputs 8 half .. 8 half + 7 of k-block 0 and
+                // 16 + 8 half .. of k-block 1
+                float f[sphsur::kUsedFeatures];
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
+                for (int i = 0; i < (int)sphsur::kUsedFeatures; ++i) f[i] = 0.0f;
                 if (row < rows) {
-                    in0[nt][kb] = *reinterpret_cast<const bf16x8*>(x + (size_t)row * 64u + (uint32_t)kb * 16u + 8u * half);
-                } else {
+                    const size_t T = in.total;
+                    sphsur::neighbourhood_features(in.pos[row].w, in.sums[row], in.sums[T + row], in.sums[2 * T + row],
+                                                   in.sums[3 * T + row], in.sums[4 * T + row], in.rho0, in.h, in.invV,
+                                                   f);
+                }
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) in0[nt][kb][i] = bits_bf16(0);
+                for (int i = 0; i < 8; ++i) {
+                    const float a = half ? f[8 + i] : f[i];
+                    const float b = (!half && 16 + i < (int)sphsur::kUsedFeatures) ? f[(16 + i) % sphsur::kUsedFeatures]
+                                                                                  : 0.0f;
+                    in0[nt][0][i] = bits_bf16(f32_to_bf16_bits(a));
+                    in0[nt][1][i] = bits_bf16(f32_to_bf16_bits(b));
+                }
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < KB0; ++kb) {
+                    if (row < rows) {
+                        in0[nt][kb] =
+                            *reinterpret_cast<const bf16x8*>(in.x + (size_t)row * 64u + (uint32_t)kb * 16u + 8u * half);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) in0[nt][kb][i] = bits_bf16(0);
+                    }
                 }
             }
         }
         bf16x8 h1[2][8], h2[2][8];
-        hidden_layer<4>(sw, sb, lane, in0, h1);
-        hidden_layer<8>(sw + kOff1, sb + 128, lane, h1, h2);
-        hidden_layer<8>(sw + kOff2, sb + 256, lane, h2, h1);
+        hidden_layer<KB0, kLd0, KB0>(sw, sb, lane, in0, h1);
+        hidden_layer<8, kLdH, 8>(sw + kOff1, sb + 128, lane, h1, h2);
+        hidden_layer<8, kLdH, 8>(sw + kOff2, sb + 256, lane, h2, h1);
         // output layer: 32 (3 used) outputs, one tile per row group
         f32x16 acc[2] = {f32x16{}, f32x16{}};
 #pragma unroll
@@ -178,7 +217,26 @@ hipError_t launch_mlp_fused(const uint16_t* x, uint32_t rows, const uint16_t* wB
                             uint32_t blocks, hipStream_t s) {
     if (rows == 0) return hipSuccess;
     if (blocks == 0) blocks = mlp_fused_blocks(rows);
-    k_mlp_fused<<<blocks, kThreads, 0, s>>>(x, rows, wBf16, wF32, y);
+    FusedInput in{};
+    in.x = x;
+    k_mlp_fused<false><<<blocks, kThreads, 0, s>>>(in, rows, wBf16, wF32, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_fused_sums(const float4* pos, const float4* sums, uint32_t total, uint32_t rows, float rho0,
+                                 float h, float invV, const uint16_t* wBf16, const float* wF32, float4* y,
+                                 uint32_t blocks, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    if (rows > total) return hipErrorInvalidValue;
+    if (blocks == 0) blocks = mlp_fused_blocks(rows);
+    FusedInput in{};
+    in.pos = pos;
+    in.sums = sums;
+    in.total = total;
+    in.rho0 = rho0;
+    in.h = h;
+    in.invV = invV;
+    k_mlp_fused<true><<<blocks, kThreads, 0, s>>>(in, rows, wBf16, wF32, y);
     return hipGetLastError();
 }
 

@@ -18,12 +18,55 @@ constexpr float kAccelScale = 98.1f;     // m/s^2: targets and outputs are in un
 constexpr float kDensityScale = 100.0f;  // density deviations and gradients enter in percent
 constexpr float kLengthScale = 0.01f;    // m: the kernel-radius feature is h / 1 cm
 
+constexpr uint32_t kUsedFeatures = 20;
+
+// The neighbourhood features of one particle (the first kUsedFeatures of its row; the
+// rest are 0), from its density and its five rows of neighbourhood sums (NeighbourSums,
+// sph_device.h):
+//   0       : 100 (rho / rho0 - 1)
+//   1 .. 3  : sum s^2 e            (shape of the density gradient)
+//   4 .. 9  : sum s^3 e e^T        (xx, yy, zz, xy, xz, yz: the neighbourhood's anisotropy)
+//   10 .. 12: sum s w / vScale     (smoothed velocity difference: the viscous pull)
+//   13      : sum s^2 (w . e) / vScale   (compression rate)
+//   14 .. 16: sum s^2 (w x e) / vScale   (rotation)
+//   17      : in-range count / 32
+//   18      : sum s / 8
+//   19      : h / kLengthScale     (one network serves several resolutions)
+// All are relative to the particle (translation invariant); vectors are in world axes,
+// so the network also sees which way gravity points.
+__device__ __forceinline__ void neighbourhood_features(float rho, float4 r0, float4 r1, float4 r2, float4 r3,
+                                                       float4 r4, float rho0, float h, float invV,
+                                                       float (&f)[kUsedFeatures]) {
+    f[0] = (rho / rho0 - 1.0f) * kDensityScale;
+    f[1] = r0.x;
+    f[2] = r0.y;
+    f[3] = r0.z;
+    f[4] = r1.x;
+    f[5] = r1.y;
+    f[6] = r1.z;
+    f[7] = r1.w;
+    f[8] = r2.x;
+    f[9] = r2.y;
+    f[10] = r3.x * invV;
+    f[11] = r3.y * invV;
+    f[12] = r3.z * invV;
+    f[13] = r2.z * invV;
+    f[14] = r4.x * invV;
+    f[15] = r4.y * invV;
+    f[16] = r4.z * invV;
+    f[17] = r0.w * (1.0f / 32.0f);
+    f[18] = r2.w * (1.0f / 8.0f);
+    f[19] = h / kLengthScale;
+}
+
 hipError_t launch_features(const sphdev::Launch& L, const float4* pos, const float4* sums, float vScale, uint16_t* x,
                            hipStream_t s);
 hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, const float4* vel, const float* y,
                                   uint32_t ldY, float aScale, uint32_t substeps, const float4* sums,
-                                  float smoothing, float4* posOut, float4* velOut, hipStream_t s);
-hipError_t launch_deviation(const sphdev::Launch& L, const float4* pos, float2* partial, hipStream_t s);
+                                  float smoothing, float4* posOut, float4* velOut, float4* posKeep,
+                                  float4* velKeep, hipStream_t s);
+hipError_t launch_deviation(const sphdev::Launch& L, const float4* pos, float2* partial, double* out, double invCount,
+                            hipStream_t s);
 hipError_t launch_gather_rows(const uint16_t* x, const uint32_t* ids, const uint32_t* slots, uint32_t count,
                               uint16_t* dst, uint32_t* dstIds, hipStream_t s);
 hipError_t launch_accumulate_accel(const float4* force, const float4* pos, const uint32_t* ids, uint32_t n,

@@ -69,9 +69,18 @@ struct SphNet {
     uint16_t *ix = nullptr, *ia = nullptr, *ib = nullptr;
     float* iy = nullptr;
     float4* sums = nullptr;   // neighbourhood sums, kSumRows x icap
+    float4 *keepPos = nullptr, *keepVel = nullptr;   // state before the last macro-step (sorted order)
     float2* devPartial = nullptr;
+    double* hostCheck = nullptr;       // pinned, mapped: the check's two means
+    double* hostCheckDev = nullptr;    // its device address
     uint32_t devPartialCap = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    // 0..3: begin/end of the network pass of prediction slots 0 and 1; 4: density check copied
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t predictedSlot = 0;   // event pair of the prediction held in iy
+    // Engine epochs (unique in the process): the solver's sorted buffers and `sums` hold
+    // the prepared state of epoch `preparedEpoch`, and `iy` the network's prediction for
+    // the prepared state of epoch `predictedEpoch` (0: none).
+    uint64_t preparedEpoch = 0, predictedEpoch = 0;
     float smoothing = 0.0f;   // XSPH velocity smoothing at the start of a macro-step (0: off)
     uint32_t maxBackoff = 0;  // after a failed check, macro-steps run plain before the next try
     uint32_t backoff = 0, skip = 0;
@@ -80,12 +89,14 @@ struct SphNet {
 
     ~SphNet() {
         hipSetDevice(device);
-        if (stream) hipStreamSynchronize(stream);
+        // rollouts queue work that uses these buffers on the solver's stream
+        hipDeviceSynchronize();
         for (void* p : {(void*)w, (void*)m1, (void*)m2, (void*)grad, (void*)frozen, (void*)wb, (void*)x, (void*)h[0],
                         (void*)h[1], (void*)h[2], (void*)dy, (void*)dh[0], (void*)dh[1], (void*)y, (void*)t,
                         (void*)partial, (void*)rowErr, (void*)order, (void*)ix, (void*)ia, (void*)ib, (void*)iy,
-                        (void*)sums, (void*)devPartial})
+                        (void*)sums, (void*)keepPos, (void*)keepVel, (void*)devPartial})
             hipFree(p);
+        if (hostCheck) hipHostFree(hostCheck);
         for (hipEvent_t e : ev)
             if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
@@ -137,7 +148,11 @@ int net_ensure_batch(SphNet* n, uint32_t rowsPad) {
 
 int net_ensure_inference(SphNet* n, uint32_t rowsPad) {
     if (rowsPad <= n->icap) return SPH_OK;
-    for (void* p : {(void*)n->ix, (void*)n->ia, (void*)n->ib, (void*)n->iy, (void*)n->sums}) hipFree(p);
+    for (void* p : {(void*)n->ix, (void*)n->ia, (void*)n->ib, (void*)n->iy, (void*)n->sums, (void*)n->keepPos,
+                    (void*)n->keepVel})
+        hipFree(p);
+    n->preparedEpoch = 0;
+    n->predictedEpoch = 0;
     const size_t R = rowsPad;
     SPH_HIP(hipMalloc(&n->ix, R * 64 * 2));
     SPH_HIP(hipMemset(n->ix, 0, R * 64 * 2));   // padded rows stay zero
@@ -145,6 +160,10 @@ int net_ensure_inference(SphNet* n, uint32_t rowsPad) {
     SPH_HIP(hipMalloc(&n->ib, R * 128 * 2));
     SPH_HIP(hipMalloc(&n->iy, R * 32 * 4));
     SPH_HIP(hipMalloc(&n->sums, R * sphdev::kSumRows * sizeof(float4)));
+    SPH_HIP(hipMalloc(&n->keepPos, R * sizeof(float4)));
+    SPH_HIP(hipMalloc(&n->keepVel, R * sizeof(float4)));
+    SPH_HIP(hipMemset(n->keepPos, 0, R * sizeof(float4)));
+    SPH_HIP(hipMemset(n->keepVel, 0, R * sizeof(float4)));
     n->icap = rowsPad;
     return SPH_OK;
 }
@@ -275,6 +294,7 @@ int net_adam(SphNet* n, float lr) {
     const float c2 = (float)(1.0 - pow(b2, (double)n->adamStep));
     SPH_HIP(sphmlp::launch_adam(n->w, n->m1, n->m2, n->grad, kParams, lr, 0.9f, 0.999f, 1e-8f, c1, c2, n->wb, n->frozen,
                                 n->stream));
+    n->predictedEpoch = 0;   // the weights changed
     return SPH_OK;
 }
 
@@ -299,6 +319,7 @@ int upload_params(SphNet* n, const float* host) {
     SPH_HIP(sphmlp::launch_to_bf16(n->w, kParams, n->wb, nullptr));
     SPH_HIP(hipDeviceSynchronize());
     n->adamStep = 0;
+    n->predictedEpoch = 0;
     return SPH_OK;
 }
 
@@ -605,26 +626,59 @@ extern "C" int sph_net_evaluate(SphNet* n, SphDataset* d, uint32_t begin, uint32
 
 namespace {
 
-int deviation_of(SphNet* n, sphhost::Engine& e, double out[2]) {
+// Enqueues the density statistics of the prepared state (reduced on the device into
+// pinned host memory), then event 4.
+int launch_check(SphNet* n, sphhost::Engine& e) {
     const sphdev::Launch L = e.plan_launch();
+    if (!n->hostCheck) {
+        SPH_HIP(hipHostMalloc(&n->hostCheck, 2 * sizeof(double), hipHostMallocMapped));
+        SPH_HIP(hipHostGetDevicePointer((void**)&n->hostCheckDev, n->hostCheck, 0));
+        SPH_HIP(hipDeviceSynchronize());
+    }
     if (n->devPartialCap < L.blocks) {
         hipFree(n->devPartial);
+        n->devPartial = nullptr;
+        n->devPartialCap = 0;
         SPH_HIP(hipMalloc(&n->devPartial, (size_t)L.blocks * sizeof(float2)));
         n->devPartialCap = L.blocks;
     }
-    SPH_HIP(sphsur::launch_deviation(L, e.sorted_pos(), n->devPartial, e.stream));
-    std::vector<float2> part(L.blocks);
-    SPH_HIP(hipMemcpyAsync(part.data(), n->devPartial, (size_t)L.blocks * sizeof(float2), hipMemcpyDeviceToHost,
-                           e.stream));
-    SPH_HIP(hipStreamSynchronize(e.stream));
-    double a = 0.0, b = 0.0;
-    for (const float2& q : part) {
-        a += q.x;
-        b += q.y;
-    }
-    const double cnt = (double)e.sc[0].n;
-    out[0] = cnt > 0 ? a / cnt : 0.0;
-    out[1] = cnt > 0 ? b / cnt : 0.0;
+    n->hostCheck[0] = 0.0;
+    n->hostCheck[1] = 0.0;
+    const uint32_t count = e.sc[0].n;
+    SPH_HIP(sphsur::launch_deviation(L, e.sorted_pos(), n->devPartial, n->hostCheckDev,
+                                     count ? 1.0 / (double)count : 0.0, e.stream));
+    SPH_HIP(hipEventRecord(n->ev[4], e.stream));
+    return SPH_OK;
+}
+
+// Waits for the check enqueued last: out[0] = mean max(rho / rho0 - 1, 0), out[1] = mean
+// |rho / rho0 - 1| (FP32 block sums added in FP64 in block order).
+int finish_check(SphNet* n, sphhost::Engine& e, double out[2]) {
+    (void)e;
+    SPH_HIP(hipEventSynchronize(n->ev[4]));
+    out[0] = n->hostCheck[0];
+    out[1] = n->hostCheck[1];
+    return SPH_OK;
+}
+
+int deviation_of(SphNet* n, sphhost::Engine& e, double out[2]) {
+    if (int rc = launch_check(n, e)) return rc;
+    return finish_check(n, e, out);
+}
+
+// The network pass on the prepared state into iy, its feature rows built in the kernel
+// from the neighbourhood sums, bracketed by the events of prediction slot `slot`.
+int predict(SphNet* n, sphhost::Engine& e, uint32_t slot) {
+    const sphdev::Launch L = e.plan_launch();
+    const SphParams& p = e.sc[0].params;
+    hipStream_t s = e.stream;
+    SPH_HIP(hipEventRecord(n->ev[2 * slot], s));
+    SPH_HIP(sphmlp::launch_mlp_fused_sums(e.sorted_pos(), n->sums, L.total, e.sc[0].n, p.restDensity,
+                                          p.smoothingRadius, 1.0f / sphsur::kVelocityScale, n->wb, n->w,
+                                          reinterpret_cast<float4*>(n->iy), 0, s));
+    SPH_HIP(hipEventRecord(n->ev[2 * slot + 1], s));
+    n->predictedEpoch = e.epoch();
+    n->predictedSlot = slot;
     return SPH_OK;
 }
 
@@ -660,9 +714,16 @@ extern "C" int sph_features(SphSolver* solver, const SphParams* p, float dtSub, 
 
 // FluidSolver::step with the surrogate: each run of 4 substeps becomes one network
 // evaluation (neighbourhood-sum features -> MLP -> 4 substeps with the predicted SPH
-// acceleration), followed by the density pass of the next substep as the check; if the mean density
-// excess max(rho / rho0 - 1, 0) exceeds `threshold`, the macro-step is redone with the
-// plain path. Substeps left over (substeps % 4) and scenes with colliders run plain.
+// acceleration), followed by the density pass of the next substep as the check; if the
+// mean density excess max(rho / rho0 - 1, 0) exceeds `threshold`, the macro-step is redone
+// with the plain path. Substeps left over (substeps % 4) and scenes with colliders run plain.
+//
+// The check's density pass is also the next macro-step's: the prepared state, and the
+// network's prediction on it, are kept with the solver's epoch and re-used by the next call
+// when nothing changed the solver in between. While the host waits for the check, the
+// prediction for the next macro-step is already queued behind it (discarded if the check
+// fails), so the GPU does not idle on the host round trip; that includes one prediction
+// queued at the end of a call for the call after it.
 extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, float deltaSeconds, uint32_t steps,
                            float threshold, SphRolloutStats* stats) {
     sphhost::clear_error();
@@ -685,8 +746,8 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
     const uint32_t R = pad32(e.total_slots());
     if (int rc = net_ensure_inference(net, R)) return rc;
     hipStream_t s = e.stream;
-    bool prepared = false;
     const sphdev::NeighbourSums ns{net->sums};
+    const size_t stateBytes = (size_t)e.total_slots() * sizeof(float4);
     for (uint32_t step = 0; step < steps; ++step) {
         for (uint32_t m = 0; m < macros; ++m) {
             if (net->skip > 0) {   // backing off after a failed check: a plain macro-step
@@ -695,27 +756,31 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
                     if (int rc = e.step(dtSub, true)) return rc;
                 st.plainSubsteps += sphsur::kMacro;
                 st.backedOff += 1;
-                prepared = false;
                 continue;
             }
-            if (!prepared) {
+            if (net->preparedEpoch != e.epoch()) {
                 if (int rc = e.prepare(dtSub, &ns)) return rc;
+                net->preparedEpoch = e.epoch();
             }
+            if (net->predictedEpoch != e.epoch())
+                if (int rc = predict(net, e, 1u - net->predictedSlot)) return rc;
+            const uint32_t used = net->predictedSlot;
             const sphdev::Launch L = e.plan_launch();
-            SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), net->sums, sphsur::kVelocityScale, net->ix, s));
-            SPH_HIP(hipEventRecord(net->ev[0], s));
-            SPH_HIP(sphmlp::launch_mlp_fused(net->ix, n, net->wb, net->w, reinterpret_cast<float4*>(net->iy), 0, s));
-            SPH_HIP(hipEventRecord(net->ev[1], s));
-            if (int rc = e.backup_state()) return rc;
+            // four substeps from the prepared (sorted) state; the state before them is kept
             SPH_HIP(sphsur::launch_integrate_macro(L, e.sorted_pos(), e.sorted_vel(), net->iy, 4, sphsur::kAccelScale,
                                                    sphsur::kMacro, net->sums, net->smoothing, e.state_pos(),
-                                                   e.state_vel(), s));
+                                                   e.state_vel(), net->keepPos, net->keepVel, s));
             e.commit_sorted();
             if (int rc = e.prepare(dtSub, &ns)) return rc;
+            net->preparedEpoch = e.epoch();
+            if (int rc = launch_check(net, e)) return rc;
+            // the next macro-step's prediction, queued before the host waits for the check
+            if (m + 1 < macros || rest == 0)
+                if (int rc = predict(net, e, 1u - used)) return rc;
             double dev[2];
-            if (int rc = deviation_of(net, e, dev)) return rc;
+            if (int rc = finish_check(net, e, dev)) return rc;
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, net->ev[0], net->ev[1]) == hipSuccess) {
+            if (hipEventElapsedTime(&ms, net->ev[2 * used], net->ev[2 * used + 1]) == hipSuccess) {
                 net->mlpMs += ms;
                 net->mlpRows += n;
             }
@@ -723,26 +788,27 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             st.lastDeviation = (float)dev[0];
             if ((float)dev[0] > st.maxDeviation) st.maxDeviation = (float)dev[0];
             if (dev[0] > (double)threshold) {
-                // The check failed: redo this macro-step with the plain path.
-                if (int rc = e.restore_state()) return rc;
+                // The check failed: back to the kept state, and the plain path redoes the
+                // macro-step. The kept state is in the prepared (sorted) order, which is the
+                // order of the committed ids.
+                SPH_HIP(hipMemcpyAsync(e.state_pos(), net->keepPos, stateBytes, hipMemcpyDeviceToDevice, s));
+                SPH_HIP(hipMemcpyAsync(e.state_vel(), net->keepVel, stateBytes, hipMemcpyDeviceToDevice, s));
+                e.touch();
                 for (uint32_t k = 0; k < sphsur::kMacro; ++k)
                     if (int rc = e.step(dtSub, true)) return rc;
                 st.fallbacks += 1;
                 st.plainSubsteps += sphsur::kMacro;
-                prepared = false;
                 if (net->maxBackoff > 0) {
                     net->backoff = net->backoff ? std::min(2 * net->backoff, net->maxBackoff) : 1u;
                     net->skip = net->backoff;
                 }
             } else {
-                prepared = true;
                 net->backoff = 0;
             }
         }
         for (uint32_t k = 0; k < rest; ++k) {
             if (int rc = e.step(dtSub, true)) return rc;
             st.plainSubsteps += 1;
-            prepared = false;
         }
     }
     return SPH_OK;

@@ -24,20 +24,8 @@ namespace {
 
 __device__ inline float clampf_ref(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
-// Features of particle t (slot order of the prepared state), from the neighbourhood
-// sums of the density pass (NeighbourSums, sph_device.h). Row layout, 64 BF16:
-//   0       : 100 (rho / rho0 - 1)
-//   1 .. 3  : sum s^2 e            (shape of the density gradient)
-//   4 .. 9  : sum s^3 e e^T        (xx, yy, zz, xy, xz, yz: the neighbourhood's anisotropy)
-//   10 .. 12: sum s w / vScale     (smoothed velocity difference: the viscous pull)
-//   13      : sum s^2 (w . e) / vScale   (compression rate)
-//   14 .. 16: sum s^2 (w x e) / vScale   (rotation)
-//   17      : in-range count / 32
-//   18      : sum s / 8
-//   19      : h / kLengthScale     (one network serves several resolutions)
-//   20 .. 63: 0
-// All are relative to the particle (translation invariant); vectors are in world axes,
-// so the network also sees which way gravity points.
+// Feature rows (neighbourhood_features, zero padded to kFeatures BF16) of the prepared
+// state, in slot order.
 __global__ void __launch_bounds__(kBlock)
     k_features(Launch L, const float4* __restrict__ pos, const float4* __restrict__ sums, float vScale,
                uint16_t* __restrict__ x) {
@@ -45,32 +33,13 @@ __global__ void __launch_bounds__(kBlock)
     if (!w.valid) return;
     const StepConsts k = w.d->k;
     const uint32_t t = w.slot;
-    const float4 r0 = sums[t], r1 = sums[(size_t)L.total + t], r2 = sums[2 * (size_t)L.total + t],
-                 r3 = sums[3 * (size_t)L.total + t], r4 = sums[4 * (size_t)L.total + t];
-    const float invV = 1.0f / vScale;
+    float used[kUsedFeatures];
+    neighbourhood_features(pos[t].w, sums[t], sums[(size_t)L.total + t], sums[2 * (size_t)L.total + t],
+                           sums[3 * (size_t)L.total + t], sums[4 * (size_t)L.total + t], k.restDensity, k.h,
+                           1.0f / vScale, used);
     float f[kFeatures];
 #pragma unroll
-    for (int i = 0; i < kFeatures; ++i) f[i] = 0.0f;
-    f[0] = (pos[t].w / k.restDensity - 1.0f) * kDensityScale;
-    f[1] = r0.x;
-    f[2] = r0.y;
-    f[3] = r0.z;
-    f[4] = r1.x;
-    f[5] = r1.y;
-    f[6] = r1.z;
-    f[7] = r1.w;
-    f[8] = r2.x;
-    f[9] = r2.y;
-    f[10] = r3.x * invV;
-    f[11] = r3.y * invV;
-    f[12] = r3.z * invV;
-    f[13] = r2.z * invV;
-    f[14] = r4.x * invV;
-    f[15] = r4.y * invV;
-    f[16] = r4.z * invV;
-    f[17] = r0.w * (1.0f / 32.0f);
-    f[18] = r2.w * (1.0f / 8.0f);
-    f[19] = k.h / kLengthScale;
+    for (int i = 0; i < kFeatures; ++i) f[i] = i < (int)kUsedFeatures ? used[i] : 0.0f;
     uint4* row = reinterpret_cast<uint4*>(x + (size_t)t * kFeatures);
 #pragma unroll
     for (int c = 0; c < kFeatures / 8; ++c) {
@@ -114,17 +83,22 @@ __device__ void field_accel(const SphForceField& fd, const float p[3], float out
 }
 
 // `substeps` substeps with the network's SPH acceleration held fixed; gravity, fields,
-// the speed clamp and the walls are applied each substep as the plain step does.
+// the speed clamp and the walls are applied each substep as the plain step does. When
+// posKeep / velKeep are set, the state the substeps start from is written there too.
 __global__ void __launch_bounds__(kBlock)
     k_integrate_macro(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
                       const float* __restrict__ y, uint32_t ldY, float aScale, uint32_t substeps,
                       const float4* __restrict__ sums, float smoothing, float4* __restrict__ posOut,
-                      float4* __restrict__ velOut) {
+                      float4* __restrict__ velOut, float4* __restrict__ posKeep, float4* __restrict__ velKeep) {
     const Where w = locate(L);
     if (!w.valid) return;
     const StepConsts k = w.d->k;
     const uint32_t t = w.slot;
     const float4 p0 = pos[t], v0 = vel[t];
+    if (posKeep) {
+        posKeep[t] = p0;
+        velKeep[t] = v0;
+    }
     const float asph[3] = {y[(size_t)t * ldY + 0] * aScale, y[(size_t)t * ldY + 1] * aScale,
                            y[(size_t)t * ldY + 2] * aScale};
     const BoundaryConsts& bnd = L.bounds[L.blockScene[sphdev::xcd_block(blockIdx.x, gridDim.x)]];
@@ -173,7 +147,7 @@ __global__ void __launch_bounds__(kBlock)
     velOut[t] = make_float4(vv[0], vv[1], vv[2], v0.w);
 }
 
-// Per-block sums of max(rho / rho0 - 1, 0) and |rho / rho0 - 1| over the prepared state.
+// Per-block FP32 sums of max(rho / rho0 - 1, 0) and |rho / rho0 - 1| over the prepared state.
 __global__ void __launch_bounds__(kBlock)
     k_deviation(Launch L, const float4* __restrict__ pos, float2* __restrict__ partial) {
     __shared__ float sa[kBlock], sb[kBlock];
@@ -195,6 +169,34 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = make_float2(sa[0], sb[0]);
+}
+
+// One block: the block sums added in FP64 in a fixed order, scaled by invCount, into
+// out[0], out[1] (host-mapped memory, read once the launch has completed).
+__global__ void __launch_bounds__(kBlock)
+    k_deviation_total(const float2* __restrict__ partial, uint32_t blocks, double invCount, double* __restrict__ out) {
+    __shared__ double da[kBlock], db[kBlock];
+    double ta = 0.0, tb = 0.0;
+    for (uint32_t i = threadIdx.x; i < blocks; i += kBlock) {
+        const float2 q = partial[i];
+        ta += (double)q.x;
+        tb += (double)q.y;
+    }
+    da[threadIdx.x] = ta;
+    db[threadIdx.x] = tb;
+    __syncthreads();
+    for (uint32_t s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            da[threadIdx.x] += da[threadIdx.x + s];
+            db[threadIdx.x] += db[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = da[0] * invCount;
+        out[1] = db[0] * invCount;
+        __threadfence_system();
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -285,16 +287,18 @@ hipError_t launch_features(const Launch& L, const float4* pos, const float4* sum
 
 hipError_t launch_integrate_macro(const Launch& L, const float4* pos, const float4* vel, const float* y, uint32_t ldY,
                                   float aScale, uint32_t substeps, const float4* sums, float smoothing,
-                                  float4* posOut, float4* velOut, hipStream_t s) {
+                                  float4* posOut, float4* velOut, float4* posKeep, float4* velKeep, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
     k_integrate_macro<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, y, ldY, aScale, substeps, sums, smoothing, posOut,
-                                                  velOut);
+                                                  velOut, posKeep, velKeep);
     return hipGetLastError();
 }
 
-hipError_t launch_deviation(const Launch& L, const float4* pos, float2* partial, hipStream_t s) {
+hipError_t launch_deviation(const Launch& L, const float4* pos, float2* partial, double* out, double invCount,
+                            hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
     k_deviation<<<L.blocks, kBlock, 0, s>>>(L, pos, partial);
+    k_deviation_total<<<1, kBlock, 0, s>>>(partial, L.blocks, invCount, out);
     return hipGetLastError();
 }
 

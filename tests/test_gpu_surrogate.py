@@ -14,7 +14,10 @@ Bars:
     restatement of the density pass's neighbourhood sums.
   * rollout: a zero threshold makes every macro-step fall back, which must reproduce
     the plain path bit for bit; a zero network must equal the restated gravity-only
-    macro-step bit for bit.
+    macro-step bit for bit; a rollout split over calls (which re-uses the prepared state
+    and the prediction queued at the end of the previous call) equals one call bit for
+    bit, and a state load, a plain step or new weights between calls are never served a
+    stale prediction.
 """
 import ctypes as C
 import os
@@ -233,6 +236,24 @@ def test_training_targets_match_the_oracle(sph, oracle_lib):
     assert np.array_equal(ref.particles()[:, :3], s.particles()[:, :3])
 
 
+def test_density_check_equals_the_oracle(sph, oracle_lib):
+    """The rollout's density check (reduced on the device): the means of
+    max(rho / rho0 - 1, 0) and |rho / rho0 - 1| over the oracle's densities, within the
+    FP32 block sums' rounding (1e-5 relative), twice in a row (the block counter resets)."""
+    p, pos, vel, s = _small_scene(seed=4)
+    dt_sub = float(F32(F32(1.0 / 600.0) / F32(8)))
+    rho, _ = _oracle_density(oracle_lib, p, pos, vel, dt_sub)
+    d = rho / F32(p.restDensity) - F32(1)
+    want = (float(np.maximum(d, 0).astype(np.float64).mean()), float(np.abs(d).astype(np.float64).mean()))
+    net = surrogate.SurrogateNet(seed=3)
+    for _ in range(2):
+        got = surrogate.density_deviation(s, net, p, dt_sub)
+        say(f"check {got} oracle {want}")
+        assert got[0] == pytest.approx(want[0], rel=1e-5, abs=1e-9)
+        assert got[1] == pytest.approx(want[1], rel=1e-5, abs=1e-9)
+    assert want[1] > 0
+
+
 def test_rollout_with_every_check_failing_is_the_plain_path(sph):
     p, pos, vel, s = _small_scene(seed=5)
     say("rollout threshold<0 == plain")
@@ -247,9 +268,70 @@ def test_rollout_with_every_check_failing_is_the_plain_path(sph):
     assert np.array_equal(a[0][:, :3], b[0][:, :3]) and np.array_equal(a[1][:, :3], b[1][:, :3])
 
 
-def _macro_gravity_only(p, pos, vel, dt_sub, max_speed):
+def _trained_net(params=None):
+    net = surrogate.SurrogateNet(seed=3)
+    net.load(surrogate.default_weights())
+    if params is not None:
+        net.params = params
+    net.set_smoothing(0.6)
+    return net
+
+
+def _same_state(a, b):
+    (pa, va), (pb, vb) = a.state(), b.state()
+    return np.array_equal(pa[:, :3], pb[:, :3]) and np.array_equal(va[:, :3], vb[:, :3])
+
+
+def test_rollout_split_over_calls_equals_one_call(sph):
+    p, pos, vel, a = _small_scene(seed=7)
+    say("rollout 4 x 1 step == 1 x 4 steps")
+    b = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    b.load_state(pos, vel)
+    na, nb = _trained_net(), _trained_net()
+    for _ in range(4):
+        surrogate.rollout(a, na, p, 1.0 / 600.0, steps=1, threshold=float("inf"))
+    st = surrogate.rollout(b, nb, p, 1.0 / 600.0, steps=4, threshold=float("inf"))
+    assert st.macroSteps == 8 and st.fallbacks == 0
+    assert _same_state(a, b)
+    pa, _ = a.state()
+    assert not np.array_equal(pa[:, :3], pos[:, :3])
+
+
+def test_rollout_is_never_served_a_stale_prediction(sph):
+    p, pos, vel, a = _small_scene(seed=8)
+    _, pos2, vel2, _ = _small_scene(seed=9)
+    say("rollout after load_state / new weights / plain step")
+    inf, dt = float("inf"), 1.0 / 600.0
+    net = _trained_net()
+    surrogate.rollout(a, net, p, dt, steps=1, threshold=inf)   # leaves a prediction queued
+    a.load_state(pos2, vel2)
+    surrogate.rollout(a, net, p, dt, steps=1, threshold=inf)
+    c = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    c.load_state(pos2, vel2)
+    surrogate.rollout(c, _trained_net(), p, dt, steps=1, threshold=inf)
+    assert _same_state(a, c)
+    # new weights between two calls
+    w = net.params * F32(0.5)
+    net.params = w
+    net.set_smoothing(0.6)
+    surrogate.rollout(a, net, p, dt, steps=1, threshold=inf)
+    other = _trained_net(w)
+    surrogate.rollout(c, other, p, dt, steps=1, threshold=inf)
+    assert _same_state(a, c)
+    # a plain step between two calls
+    a.step(p, dt)
+    c.step(p, dt)
+    surrogate.rollout(a, net, p, dt, steps=1, threshold=inf)
+    surrogate.rollout(c, _trained_net(w), p, dt, steps=1, threshold=inf)
+    assert _same_state(a, c)
+
+
+def _macro_gravity_only(p, pos, vel, dt_sub, max_speed, accel=None):
+    """One macro-step restated: 4 substeps holding the SPH acceleration `accel` (0 when
+    None) with gravity, the speed clamp and the walls, in the kernel's FP32 order."""
     x = pos[:, :3].astype(F32).copy()
     v = vel[:, :3].astype(F32).copy()
+    a_sph = np.zeros_like(v) if accel is None else np.asarray(accel, F32)
     g = np.array(p.gravity, F32)
     r = F32(p.particleRadius)
     low = np.array(p.containerMin, F32) + r
@@ -257,7 +339,7 @@ def _macro_gravity_only(p, pos, vel, dt_sub, max_speed):
     dt = F32(dt_sub)
     e = F32(p.wallRestitution)
     for _ in range(4):
-        v = v + ((F32(0) + g) + F32(0)) * dt
+        v = v + ((a_sph + g) + F32(0)) * dt
         speed = np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
         over = speed > max_speed
         sc = np.where(over, max_speed / np.where(over, speed, 1), 1).astype(F32)
@@ -269,6 +351,34 @@ def _macro_gravity_only(p, pos, vel, dt_sub, max_speed):
             v[:, a] = np.where(lo & (v[:, a] < 0), -v[:, a] * e, v[:, a])
             v[:, a] = np.where(hi & (v[:, a] > 0), -v[:, a] * e, v[:, a])
     return x, v
+
+
+def test_rollout_equals_features_forward_and_restated_macro_steps(sph):
+    """The rollout builds its feature rows inside the fused network kernel; restated
+    with the separate kernels instead (sph_features -> sph_net_forward) and the macro-step
+    in numpy, two macro-steps of the trained network match bit for bit."""
+    p, pos, vel, s = _small_scene(seed=10)
+    say("rollout == features + forward + restated macro-steps")
+    net = surrogate.SurrogateNet(seed=3)
+    net.load(surrogate.default_weights())
+    dt_sub = F32(F32(1.0 / 600.0) / F32(8))
+    consts = (C.c_float * 8)()
+    capi.check(capi.load().sph_debug_step_constants(C.byref(p), float(dt_sub), consts))
+    scale = F32(98.1)
+    x, v = pos.copy(), vel.copy()
+    probe = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    for _ in range(2):
+        probe.load_state(x, v)
+        y = net.forward(surrogate.features(probe, p, float(dt_sub)))[:, :3]
+        assert np.abs(y).max() > 0.01
+        xn, vn = _macro_gravity_only(p, x, v, dt_sub, F32(consts[5]), accel=y.astype(F32) * scale)
+        x = np.c_[xn, np.zeros(len(xn), F32)]
+        v = np.c_[vn, np.zeros(len(vn), F32)]
+    st = surrogate.rollout(s, net, p, 1.0 / 600.0, steps=1, threshold=float("inf"))
+    assert st.macroSteps == 2 and st.fallbacks == 0
+    gp, gv = s.state()
+    np.testing.assert_array_equal(gp[:, :3], x[:, :3])
+    np.testing.assert_array_equal(gv[:, :3], v[:, :3])
 
 
 def test_zero_network_rollout_is_the_gravity_only_macro_step(sph):
