@@ -185,13 +185,15 @@ def surrogate_leg(w, steps: int) -> dict | None:
         "density_error_vs_plain": r["density_error_vs_plain"],
         "energy_drift": {"plain": r["energy_drift_plain"], "surrogate": r["energy_drift_surrogate"]},
         "unchecked": {"speedup_vs_plain": r.get("unchecked_speedup"),
+                      "ms_per_step": r.get("unchecked_ms_per_step"),
+                      "density_excess_max": max(r.get("unchecked_density_excess") or [float("nan")]),
                       "density_excess_last": (r.get("unchecked_density_excess") or [None])[-1]},
         "roofline": None,
     }
     if r.get("mlp_tflops"):
         out["roofline"] = {"bound": "mfma", "kernel": "mlp_fused", "achieved": r["mlp_tflops"], "peak": 2500.0,
                            "unit": "TFLOP/s", "frac": r["mlp_tflops"] / 2500.0,
-                           "flop_per_unit": S.MLP_FLOP_PER_ROW, "ms_per_macro_step": r["mlp_ms_per_macro_step"]}
+                           "flop_per_unit": S.ROLLOUT_MLP_FLOP_PER_ROW, "ms_per_macro_step": r["mlp_ms_per_macro_step"]}
     return out
 
 

@@ -163,6 +163,22 @@ int hip_fail(hipError_t e, const char* what);
 // The C-ABI handles are engines: a solver has one scene, a batch many.
 struct SphSolver {
     sphhost::Engine e;
+    // device buffers of the surface extraction, grown on demand and kept with the solver
+    float* surfField = nullptr;
+    size_t surfFieldBytes = 0;
+    SphSurfaceVertex* surfVerts = nullptr;
+    uint32_t surfVertCap = 0;
+    uint32_t* surfCounters = nullptr;
+
+    ~SphSolver() {
+        if (surfField || surfVerts || surfCounters) {
+            hipSetDevice(e.device);
+            hipDeviceSynchronize();
+            hipFree(surfField);
+            hipFree(surfVerts);
+            hipFree(surfCounters);
+        }
+    }
 };
 
 struct SphBatch {

@@ -227,11 +227,11 @@ int field_on_device(sphhost::Engine& e, const SphParams& p, uint32_t resolution,
     return SPH_OK;
 }
 
+// `counters`: two device words, overwritten.
 int march_on_device(const float* field, uint32_t resolution, const float origin[3], const float cellSize[3],
-                    float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written, hipStream_t s) {
+                    float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written, uint32_t* counters,
+                    hipStream_t s) {
     if (int st = upload_table()) return st;
-    uint32_t* counters = nullptr;
-    SPH_HIP(hipMalloc(&counters, 2 * sizeof(uint32_t)));
     const uint32_t init[2] = {0u, capacity};
     hipError_t he = hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, s);
     MarchArgs m;
@@ -250,9 +250,28 @@ int march_on_device(const float* field, uint32_t resolution, const float origin[
     uint32_t got[2] = {0u, 0u};
     if (he == hipSuccess) he = hipMemcpyAsync(got, counters, sizeof(got), hipMemcpyDeviceToHost, s);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
-    hipFree(counters);
     if (he != hipSuccess) return hip_fail(he, "marching cubes");
     *written = got[0] < got[1] ? got[0] : got[1];
+    return SPH_OK;
+}
+
+// The solver's field and vertex buffers, grown to at least these sizes (never shrunk).
+int ensure_surface_buffers(SphSolver* solver, size_t fieldBytes, uint32_t vertices) {
+    if (solver->surfFieldBytes < fieldBytes) {
+        hipFree(solver->surfField);
+        solver->surfField = nullptr;
+        solver->surfFieldBytes = 0;
+        SPH_HIP(hipMalloc(&solver->surfField, fieldBytes));
+        solver->surfFieldBytes = fieldBytes;
+    }
+    if (solver->surfVertCap < vertices) {
+        hipFree(solver->surfVerts);
+        solver->surfVerts = nullptr;
+        solver->surfVertCap = 0;
+        SPH_HIP(hipMalloc(&solver->surfVerts, (size_t)vertices * sizeof(SphSurfaceVertex)));
+        solver->surfVertCap = vertices;
+    }
+    if (!solver->surfCounters) SPH_HIP(hipMalloc(&solver->surfCounters, 2 * sizeof(uint32_t)));
     return SPH_OK;
 }
 
@@ -275,15 +294,13 @@ extern "C" int sph_surface_field(SphSolver* solver, const SphParams* p, uint32_t
     if (int st = e.set_device()) return st;
     const uint32_t s = resolution + 1;
     const size_t bytes = (size_t)s * s * s * sizeof(float);
-    float* d = nullptr;
-    SPH_HIP(hipMalloc(&d, bytes));
-    int st = field_on_device(e, *p, resolution, d);
+    if (int st = ensure_surface_buffers(solver, bytes, 0)) return st;
+    int st = field_on_device(e, *p, resolution, solver->surfField);
     if (!st) {
-        hipError_t he = hipMemcpyAsync(field, d, bytes, hipMemcpyDeviceToHost, e.stream);
+        hipError_t he = hipMemcpyAsync(field, solver->surfField, bytes, hipMemcpyDeviceToHost, e.stream);
         if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
         if (he != hipSuccess) st = hip_fail(he, "sph_surface_field");
     }
-    hipFree(d);
     return st;
 }
 
@@ -304,18 +321,21 @@ extern "C" int sph_march_field(const float* field, uint32_t resolution, const fl
     const size_t bytes = (size_t)s * s * s * sizeof(float);
     float* d = nullptr;
     SphSurfaceVertex* v = nullptr;
+    uint32_t* counters = nullptr;
     SPH_HIP(hipMalloc(&d, bytes));
     hipError_t he = hipMalloc(&v, (size_t)capacity * sizeof(SphSurfaceVertex));
+    if (he == hipSuccess) he = hipMalloc(&counters, 2 * sizeof(uint32_t));
     int st = SPH_OK;
     if (he == hipSuccess) he = hipMemcpy(d, field, bytes, hipMemcpyHostToDevice);
     if (he != hipSuccess) st = hip_fail(he, "sph_march_field");
-    if (!st) st = march_on_device(d, resolution, origin, cellSize, iso, v, capacity, written, nullptr);
+    if (!st) st = march_on_device(d, resolution, origin, cellSize, iso, v, capacity, written, counters, nullptr);
     if (!st && *written) {
         he = hipMemcpy(out, v, (size_t)*written * sizeof(SphSurfaceVertex), hipMemcpyDeviceToHost);
         if (he != hipSuccess) st = hip_fail(he, "sph_march_field");
     }
     hipFree(d);
     hipFree(v);
+    hipFree(counters);
     return st;
 }
 
@@ -334,23 +354,19 @@ extern "C" int sph_surface_mesh(SphSolver* solver, const SphParams* p, uint32_t 
     if (int st = e.set_params(0, *p)) return st;
     if (int st = e.set_device()) return st;
     const uint32_t s = resolution + 1;
-    float* d = nullptr;
-    SphSurfaceVertex* v = nullptr;
-    SPH_HIP(hipMalloc(&d, (size_t)s * s * s * sizeof(float)));
-    hipError_t he = hipMalloc(&v, (size_t)capacity * sizeof(SphSurfaceVertex));
-    int st = he == hipSuccess ? SPH_OK : hip_fail(he, "sph_surface_mesh");
-    if (!st) st = field_on_device(e, *p, resolution, d);
+    if (int st = ensure_surface_buffers(solver, (size_t)s * s * s * sizeof(float), capacity)) return st;
+    float* d = solver->surfField;
+    SphSurfaceVertex* v = solver->surfVerts;
+    int st = field_on_device(e, *p, resolution, d);
     float origin[3], cell[3];
     for (int k = 0; k < 3; ++k) {
         origin[k] = p->containerMin[k];
         cell[k] = (p->containerMax[k] - p->containerMin[k]) / (float)resolution;
     }
-    if (!st) st = march_on_device(d, resolution, origin, cell, iso, v, capacity, written, e.stream);
+    if (!st) st = march_on_device(d, resolution, origin, cell, iso, v, capacity, written, solver->surfCounters, e.stream);
     if (!st && *written) {
-        he = hipMemcpy(out, v, (size_t)*written * sizeof(SphSurfaceVertex), hipMemcpyDeviceToHost);
+        const hipError_t he = hipMemcpy(out, v, (size_t)*written * sizeof(SphSurfaceVertex), hipMemcpyDeviceToHost);
         if (he != hipSuccess) st = hip_fail(he, "sph_surface_mesh");
     }
-    hipFree(d);
-    hipFree(v);
     return st;
 }

@@ -22,9 +22,18 @@ from .fluid import FluidSolver
 _FP = C.POINTER(C.c_float)
 
 
-def _unpack(buf, n):
-    arr = np.frombuffer(bytes(buf), dtype=np.uint8, count=n * 16).reshape(n, 16)
-    return arr[:, :12].copy().view(np.float32).reshape(n, 3), arr[:, 12:].copy().view(np.uint32).reshape(n)
+def _vertex_buffer(capacity: int) -> np.ndarray:
+    """Room for `capacity` SphSurfaceVertex records (16 bytes: position, packed normal)."""
+    assert C.sizeof(SphSurfaceVertex) == 16
+    return np.empty((capacity, 4), np.uint32)
+
+
+def _vertex_ptr(buf: np.ndarray):
+    return buf.ctypes.data_as(C.c_void_p)
+
+
+def _unpack(buf: np.ndarray, n: int):
+    return buf[:n, :3].copy().view(np.float32), buf[:n, 3].copy()
 
 
 def build_fluid_field(solver: FluidSolver, params: SphParams, resolution: int) -> np.ndarray:
@@ -40,18 +49,18 @@ def grid_cell(params: SphParams, resolution: int) -> list[float]:
 
 
 def march_fluid_field(field: np.ndarray, resolution: int, origin, cell, iso: float, capacity: int = 1 << 22):
-    buf = (SphSurfaceVertex * capacity)()
+    buf = _vertex_buffer(capacity)
     written = C.c_uint32()
     f = np.ascontiguousarray(field, np.float32)
     check(capi.load().sph_march_field(f.ctypes.data_as(_FP), resolution, (C.c_float * 3)(*origin),
-                                      (C.c_float * 3)(*cell), iso, buf, capacity, C.byref(written)))
+                                      (C.c_float * 3)(*cell), iso, _vertex_ptr(buf), capacity, C.byref(written)))
     return _unpack(buf, written.value)
 
 
 def surface_mesh(solver: FluidSolver, params: SphParams, resolution: int, iso: float, capacity: int = 1 << 22):
-    buf = (SphSurfaceVertex * capacity)()
+    buf = _vertex_buffer(capacity)
     written = C.c_uint32()
-    check(capi.load().sph_surface_mesh(solver._h, C.byref(params), resolution, iso, buf, capacity,
+    check(capi.load().sph_surface_mesh(solver._h, C.byref(params), resolution, iso, _vertex_ptr(buf), capacity,
                                        C.byref(written)))
     return _unpack(buf, written.value)
 

@@ -1,7 +1,10 @@
 """The fluid surface on the CPU: the product's marching-cubes table against the properties
-the reference checks of its own table (tests/marching_cubes_test.cpp:120-207), and the
-surface oracle against closed forms and brute force (:209-318)."""
+the reference checks of its own table (tests/marching_cubes_test.cpp:120-207) and, where
+the reference tree is present, against that table itself (read in place as text), and
+the surface oracle against closed forms and brute force (:209-318)."""
 import ctypes as C
+import os
+import re
 
 import numpy as np
 import pytest
@@ -48,6 +51,22 @@ def outward(code, tri):
         frm, to = (a, b) if inside(code, a) else (b, a)
         total += CORNERS[to] - CORNERS[frm]
     return total
+
+
+REFERENCE_TABLE = "/root/reference/src/physics/marching_cubes.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_TABLE), reason="reference tree not present")
+def test_table_equals_the_reference_table(table):
+    """The table generated from the face-contour rule equals physics::MC_TABLE
+    (src/physics/marching_cubes.cpp:39-296) entry for entry, so the triangles are the
+    reference's, in its order within each cell."""
+    text = open(REFERENCE_TABLE).read()
+    body = text[text.index("MC_TABLE{{"):]
+    rows = re.findall(r"\{([-0-9, ]+)\}", body)[:256]
+    ref = np.array([[int(v) for v in r.split(",")] for r in rows], np.int8)
+    assert ref.shape == (256, 16)
+    assert np.array_equal(ref, table)
 
 
 def test_table_shape_and_crossed_edges(table):
