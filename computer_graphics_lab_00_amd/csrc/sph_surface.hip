@@ -48,31 +48,23 @@ __device__ inline uint32_t hash_cell(int32_t cx, int32_t cy, int32_t cz, uint32_
     return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & mask;
 }
 
-// sampleField (marching_cubes.cpp:312-343) at every interior sample; the samples on the
-// grid's faces stay 0 so the surface closes there (:380-383).
-__global__ void __launch_bounds__(kThreads)
-    k_field(const float4* __restrict__ pos, const uint32_t* __restrict__ bucketStart,
-            const uint32_t* __restrict__ counts, FieldArgs a, float* __restrict__ field) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    const uint32_t s = a.samples;
-    if (i >= s * s * s) return;
-    const uint32_t x = i % s, y = (i / s) % s, z = i / (s * s);
-    if (x == 0 || y == 0 || z == 0 || x == a.resolution || y == a.resolution || z == a.resolution) {
-        field[i] = 0.0f;
-        return;
-    }
-    const float px = a.origin[0] + (float)x * a.cell[0];
-    const float py = a.origin[1] + (float)y * a.cell[1];
-    const float pz = a.origin[2] + (float)z * a.cell[2];
-    const int32_t bx = (int32_t)floorf(px / a.radius), by = (int32_t)floorf(py / a.radius),
-                  bz = (int32_t)floorf(pz / a.radius);
+__device__ inline int32_t field_cell(float origin, float cell, uint32_t i, float radius) {
+    return (int32_t)floorf((origin + (float)i * cell) / radius);
+}
+
+// sampleField (marching_cubes.cpp:312-343) at one sample point, reading the particles from
+// the sorted global arrays: the 27 buckets around the point's cell in the reference's order
+// (z, y, x), a bucket two neighbour cells fold into summed once (:319-326), at most 32
+// particles per bucket.
+__device__ float sample_global(const float4* __restrict__ pos, const uint32_t* __restrict__ bucketStart,
+                               const uint32_t* __restrict__ counts, const FieldArgs& a, float px, float py, float pz,
+                               int32_t bx, int32_t by, int32_t bz) {
     uint32_t bk[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) bk[o] = hash_cell(bx + o % 3 - 1, by + (o / 3) % 3 - 1, bz + o / 9 - 1, a.cellMask);
     float total = 0.0f;
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
-        // a bucket two neighbour cells fold into is summed once (:319-326)
         bool dup = false;
 #pragma unroll
         for (int q = 0; q < o; ++q) dup |= bk[q] == bk[o];
@@ -88,6 +80,155 @@ __global__ void __launch_bounds__(kThreads)
             if (dsq >= a.radiusSq) continue;
             const float d = 1.0f - dsq / a.radiusSq;
             total += (d * d) * d;
+        }
+    }
+    return total;
+}
+
+// The field over tiles of kTileX x kTileY x kTileZ samples, one workgroup each: the tile's
+// samples need the cells of one box (their cells, one more on every side); the workgroup
+// copies the particles of every cell of the box (the bucket's first 32, in bucket order)
+// into LDS once, and each sample then walks its 27 cells there — same buckets, same order,
+// same arithmetic as sample_global, so the same bits — instead of re-reading every particle
+// from L2 for each of the ~200 samples whose neighbourhood covers it. A tile whose box has
+// more than kBoxCells cells or kLdsParticles particles takes the global path. The samples on
+// the grid's faces stay 0 so the surface closes there (:380-383).
+constexpr uint32_t kTileX = 8, kTileY = 8, kTileZ = 4;   // 256 samples
+constexpr uint32_t kBoxCells = 256;
+constexpr uint32_t kLdsParticles = 1024;
+constexpr uint32_t kBatch = 4;   // candidates read from LDS together
+static_assert(kTileX * kTileY * kTileZ == kThreads, "one sample per thread");
+static_assert(kBoxCells <= 4 * kThreads, "each thread scans at most four cells");
+
+__global__ void __launch_bounds__(kThreads)
+    k_field(const float4* __restrict__ pos, const uint32_t* __restrict__ bucketStart,
+            const uint32_t* __restrict__ counts, FieldArgs a, uint32_t tilesX, uint32_t tilesY,
+            float* __restrict__ field) {
+    __shared__ float4 sp[kLdsParticles + kBatch];   // a batch may read past the last cell
+    __shared__ uint32_t cellOff[kBoxCells + 1];
+    __shared__ uint32_t cellBk[kBoxCells];
+    __shared__ uint32_t scan[kThreads];
+    __shared__ int32_t box[6];   // lowest cell, extent
+    __shared__ uint32_t useLds;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t s = a.samples;
+    const uint32_t tx = blockIdx.x % tilesX, ty = (blockIdx.x / tilesX) % tilesY, tz = blockIdx.x / (tilesX * tilesY);
+    const uint32_t x = tx * kTileX + tid % kTileX, y = ty * kTileY + (tid / kTileX) % kTileY,
+                   z = tz * kTileZ + tid / (kTileX * kTileY);
+    if (tid == 0) {
+        const uint32_t lo[3] = {tx * kTileX, ty * kTileY, tz * kTileZ};
+        const uint32_t span[3] = {kTileX, kTileY, kTileZ};
+        uint64_t cells = 1u;
+        bool ok = true;
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t hi = min(lo[k] + span[k] - 1u, s - 1u);
+            // with a positive cell size floor(p / h) does not decrease along an axis: the
+            // tile's first and last samples bound it
+            const int32_t b0 = field_cell(a.origin[k], a.cell[k], lo[k], a.radius) - 1;
+            const int32_t b1 = field_cell(a.origin[k], a.cell[k], hi, a.radius) + 1;
+            const int64_t ext = (int64_t)b1 - (int64_t)b0 + 1;
+            if (!(a.cell[k] > 0.0f) || ext <= 0 || ext > (int64_t)kBoxCells) ok = false;
+            box[k] = b0;
+            box[3 + k] = ok ? (int32_t)ext : 0;
+            if (ok) cells *= (uint64_t)ext;
+        }
+        useLds = (ok && cells <= kBoxCells) ? (uint32_t)cells : 0u;
+    }
+    __syncthreads();
+    const uint32_t ncell = useLds;
+    if (ncell) {
+        // per cell: its bucket and how many of its particles are visited; offsets by a scan
+        const uint32_t per = (ncell + kThreads - 1u) / kThreads;   // <= 4
+        uint32_t local = 0u;
+        for (uint32_t j = 0; j < per; ++j) {
+            const uint32_t c = tid * per + j;
+            if (c < ncell) {
+                const int32_t cx = box[0] + (int32_t)(c % (uint32_t)box[3]);
+                const int32_t cy = box[1] + (int32_t)((c / (uint32_t)box[3]) % (uint32_t)box[4]);
+                const int32_t cz = box[2] + (int32_t)(c / ((uint32_t)box[3] * (uint32_t)box[4]));
+                const uint32_t bk = hash_cell(cx, cy, cz, a.cellMask);
+                uint32_t cnt = counts[bk];
+                if (cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+                cellBk[c] = bk;
+                cellOff[c] = cnt;
+                local += cnt;
+            }
+        }
+        scan[tid] = local;
+        __syncthreads();
+        for (uint32_t d = 1; d < kThreads; d <<= 1) {
+            const uint32_t v = tid >= d ? scan[tid - d] : 0u;
+            __syncthreads();
+            scan[tid] += v;
+            __syncthreads();
+        }
+        uint32_t run = scan[tid] - local;   // exclusive prefix of this thread's cells
+        for (uint32_t j = 0; j < per; ++j) {
+            const uint32_t c = tid * per + j;
+            if (c < ncell) {
+                const uint32_t cnt = cellOff[c];
+                cellOff[c] = run;
+                run += cnt;
+            }
+        }
+        if (tid == kThreads - 1u) cellOff[ncell] = scan[kThreads - 1u];
+        __syncthreads();
+        if (cellOff[ncell] > kLdsParticles) {
+            if (tid == 0) useLds = 0u;
+        } else {
+            // one wave per cell at a time: lane k copies the cell's k-th particle
+            const uint32_t lane = tid & 63u, wave = tid >> 6;
+            for (uint32_t c = wave; c < ncell; c += kThreads / 64u) {
+                const uint32_t n = cellOff[c + 1] - cellOff[c];
+                if (lane < n) sp[cellOff[c] + lane] = pos[bucketStart[cellBk[c]] + lane];
+            }
+        }
+        __syncthreads();
+    }
+    if (x >= s || y >= s || z >= s) return;
+    const uint32_t i = (z * s + y) * s + x;
+    if (x == 0 || y == 0 || z == 0 || x == a.resolution || y == a.resolution || z == a.resolution) {
+        field[i] = 0.0f;
+        return;
+    }
+    const float px = a.origin[0] + (float)x * a.cell[0];
+    const float py = a.origin[1] + (float)y * a.cell[1];
+    const float pz = a.origin[2] + (float)z * a.cell[2];
+    const int32_t bx = (int32_t)floorf(px / a.radius), by = (int32_t)floorf(py / a.radius),
+                  bz = (int32_t)floorf(pz / a.radius);
+    if (!useLds) {
+        field[i] = sample_global(pos, bucketStart, counts, a, px, py, pz, bx, by, bz);
+        return;
+    }
+    // the 27 cells as box indices; the box holds every one of them
+    uint32_t cidx[27], bk[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        const uint32_t cx = (uint32_t)(bx + o % 3 - 1 - box[0]), cy = (uint32_t)(by + (o / 3) % 3 - 1 - box[1]),
+                       cz = (uint32_t)(bz + o / 9 - 1 - box[2]);
+        cidx[o] = cx + (uint32_t)box[3] * (cy + (uint32_t)box[4] * cz);
+        bk[o] = cellBk[cidx[o]];
+    }
+    float total = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        bool dup = false;
+#pragma unroll
+        for (int q = 0; q < o; ++q) dup |= bk[q] == bk[o];
+        if (dup) continue;
+        const uint32_t lo = cellOff[cidx[o]], hi = cellOff[cidx[o] + 1];
+        for (uint32_t k = lo; k < hi; k += kBatch) {
+            float4 q[kBatch];
+#pragma unroll
+            for (uint32_t e = 0; e < kBatch; ++e) q[e] = sp[k + e];
+#pragma unroll
+            for (uint32_t e = 0; e < kBatch; ++e) {
+                const float ex = q[e].x - px, ey = q[e].y - py, ez = q[e].z - pz;
+                const float dsq = (ex * ex + ey * ey) + ez * ez;
+                if (k + e >= hi || dsq >= a.radiusSq) continue;
+                const float d = 1.0f - dsq / a.radiusSq;
+                total += (d * d) * d;
+            }
         }
     }
     field[i] = total;
@@ -221,8 +362,10 @@ int field_on_device(sphhost::Engine& e, const SphParams& p, uint32_t resolution,
     a.samples = s;
     a.resolution = resolution;
     a.cellMask = p.cellCount - 1u;
-    k_field<<<(uint32_t)((samples + kThreads - 1) / kThreads), kThreads, 0, e.stream>>>(
-        e.sorted_pos(), e.bucket_starts(), e.bucket_counts(), a, field);
+    const uint32_t tilesX = (s + kTileX - 1) / kTileX, tilesY = (s + kTileY - 1) / kTileY,
+                   tilesZ = (s + kTileZ - 1) / kTileZ;
+    k_field<<<tilesX * tilesY * tilesZ, kThreads, 0, e.stream>>>(e.sorted_pos(), e.bucket_starts(), e.bucket_counts(),
+                                                                 a, tilesX, tilesY, field);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
