@@ -267,26 +267,41 @@ __device__ inline uint32_t pack_unit(float nx, float ny, float nz) {
     return q(u) | (q(v) << 16);
 }
 
-// marchFluidField (marching_cubes.cpp:393-487), one lane per cell.
+// marchFluidField (marching_cubes.cpp:393-487), one lane per cell. Each wave takes the slots
+// of all its cells' vertices with one atomic (a scan of the lanes' vertex counts hands each
+// lane its range), so ranges stay contiguous and disjoint in the order the waves got them.
 __global__ void __launch_bounds__(kThreads)
     k_march(const float* __restrict__ f, MarchArgs m, SphSurfaceVertex* __restrict__ out,
             uint32_t* __restrict__ counters) {
     const uint32_t r = m.resolution, s = r + 1;
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= r * r * r) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool inside = i < r * r * r;
     const int32_t bx = (int32_t)(i % r), by = (int32_t)((i / r) % r), bz = (int32_t)(i / (r * r));
     float val[8];
     uint32_t code = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        val[c] = field_at(f, s, bx + c_corner[c][0], by + c_corner[c][1], bz + c_corner[c][2]);
+        val[c] = inside ? field_at(f, s, bx + c_corner[c][0], by + c_corner[c][1], bz + c_corner[c][2]) : 0.0f;
         if (val[c] >= m.iso) code |= 1u << c;
     }
+    if (!inside) code = 0u;
     const int8_t* entry = c_table + code * 16u;
-    if (entry[0] < 0) return;
     uint32_t nv = 0;
     while (nv < 16u && entry[nv] >= 0) ++nv;
-    const uint32_t slot = atomicAdd(&counters[0], nv);
+    // wave-wide inclusive scan of nv (every lane of the wave takes part)
+    uint32_t incl = nv;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t up = __shfl_up(incl, d, 64);
+        if (lane >= (uint32_t)d) incl += up;
+    }
+    const uint32_t waveTotal = __shfl(incl, 63, 64);
+    uint32_t base = 0u;
+    if (lane == 63u && waveTotal) base = atomicAdd(&counters[0], waveTotal);
+    base = __shfl(base, 63, 64);
+    if (nv == 0u) return;
+    const uint32_t slot = base + incl - nv;
     if (slot + nv > m.capacity) {
         atomicMin(&counters[1], slot);   // the first slot that did not fit (:467-472)
         return;
