@@ -9,7 +9,8 @@
 // inside corner off separately, and the triangles wind so their normal points from the
 // inside corners to the outside ones. This file builds the table from that rule: face
 // segments → closed loops of crossed edges → each loop oriented outward and fanned into
-// triangles. The table itself is this build's own.
+// triangles. The generator is written from that rule alone; the table it produces equals
+// the reference's MC_TABLE (marching_cubes.cpp:39-296) entry for entry.
 #include <stdint.h>
 #include <string.h>
 

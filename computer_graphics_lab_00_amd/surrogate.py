@@ -1,7 +1,7 @@
 """The learned surrogate of the SPH step, host side.
 
 No counterpart in the reference: this sits on top of ``physics::FluidSolver::step``
-(src/physics/fluid_sph.cpp:151-303) and keeps that path as its baseline and judge.
+(src/physics/fluid_sph.cpp:151-303) and keeps that path as its baseline and as the fallback it checks against.
 
 * ``SurrogateNet``: the MLP 64 -> 128 -> 128 -> 128 -> 3 (ReLU) that maps a particle's
   neighbourhood features to its SPH acceleration; trained and evaluated on the GPU by

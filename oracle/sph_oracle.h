@@ -9,7 +9,7 @@
  *   orc32_*  — FP32, operation for operation in the reference's order (glm's scalar
  *              code path: dot = (x*x + y*y) + z*z, vec/scalar = per-component IEEE
  *              division, mat4*vec4 = (m0 x + m1 y) + (m2 z + m3 w)), compiled with
- *              -ffp-contract=off and no fast-math. This is the parity judge.
+ *              -ffp-contract=off and no fast-math. This is the parity reference.
  *   orc64_*  — the same algorithm in FP64, for error statements. Its grid keys are
  *              computed with the FP32 rule on FP32-rounded positions so that both
  *              oracles see the same neighbour sets; everything else is FP64.

@@ -2,8 +2,8 @@
 
 Bar: the grid (keys, counts, slot order) bit-exact; density, pressure, force and the
 integrated state bit-exact as well (every operation is evaluated in the reference's
-order with IEEE rounding, no contraction). The north-star tolerance, 1e-5 relative on
-density and force, is asserted separately so a failure says which bar broke. Each
+order with IEEE rounding, no contraction). The 1e-5 relative tolerance on density and
+force is asserted separately so a failure says which bar broke. Each
 result's error against the FP64 oracle is printed.
 """
 import os
@@ -18,7 +18,7 @@ from tests.golden import make_golden as G
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
-REL_TOL = 1e-5   # north-star tolerance on per-particle density and force
+REL_TOL = 1e-5   # relative tolerance on per-particle density and force
 
 
 def say(msg):
@@ -77,7 +77,7 @@ def compare_substep(gpu, ora, p, label):
     gp, gv = gpu.state()
     op, ov = ora.state()
     gf, of = gpu.forces(), ora.forces()
-    # north-star tolerance first (density, pressure, force)
+    # the 1e-5 tolerance first (density, pressure, force)
     assert rel_err(gp[:, 3], op[:, 3]) <= REL_TOL, f"{label}: density beyond 1e-5"
     assert norm_err(gf[:, :3], of[:, :3]) <= REL_TOL, f"{label}: force beyond 1e-5"
     # then bit-exactness of everything
