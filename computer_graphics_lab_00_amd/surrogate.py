@@ -250,7 +250,7 @@ class TrainingScene:
         return p, pos, vel
 
 
-def training_scene(index: int, seed: int = 2024, dam_every: int = 3) -> TrainingScene:
+def training_scene(index: int, seed: int = 2024, dam_every: int = 3, dam_warmup: int = 48) -> TrainingScene:
     """Scene `index` of the seeded training family, at one of the two resolutions. One scene
     in `dam_every` is a dam break at rest — one of three 64k columns (16 x 128 x 32,
     32 x 128 x 16, 24 x 96 x 28) in the corner of a box 4-8x its length and 1-2x its depth —
@@ -283,7 +283,8 @@ def training_scene(index: int, seed: int = 2024, dam_every: int = 3) -> Training
     wl = Workload(f"train_{index}", lattice, spacing, (0.0, 0.0, 0.0), tuple(box), h,
                   cell_count=max(1024, 1 << int(math.ceil(math.log2(2 * count)))),
                   jitter_seed=(seed + index) & 0xFFFFFFFF, corner=tuple(corner))
-    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * 48), at_rest=dam)
+    warm = dam_warmup if dam else 48
+    return TrainingScene(wl, velocity_seed=seed * 7919 + index, warmup_steps=int(u[7] * warm), at_rest=dam)
 
 
 # Position noise (fraction of the lattice spacing) and velocity noise (m/s) added before
@@ -306,7 +307,7 @@ def perturb(solver: FluidSolver, spacing: float, pos_noise: float, vel_noise: fl
 
 def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4096, dt: float = 1.0 / 600.0,
              gap_steps: int = 2, seed: int = 2024, first: int = 0, noise=NOISE, net: "SurrogateNet | None" = None,
-             dam_every: int = 3, log=None) -> list[tuple[int, int]]:
+             dam_every: int = 3, dam_warmup: int = 48, log=None) -> list[tuple[int, int]]:
     """Records `records` macro-steps of each of scenes first..first+scenes-1 into `data`;
     returns each scene's row range. Between records the scene runs `gap_steps` steps so
     the rows of one scene are not all alike; before record r the state is perturbed by
@@ -317,7 +318,7 @@ def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4
     solver = None
     cap = 0
     for i in range(first, first + scenes):
-        sc = training_scene(i, seed, dam_every)
+        sc = training_scene(i, seed, dam_every, dam_warmup)
         p, pos, vel = sc.initial_state()
         n = pos.shape[0]
         if solver is None or n > cap or sc.workload.cell_count > solver.max_cell_count:
@@ -377,7 +378,8 @@ def record_workload(data: TrainingSet, workload: Workload, records: int, samples
 
 def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, records: int = 12,
                   samples: int = 4096, steps: int = 6000, batch: int = 8192, lr: float = 1e-3,
-                  eval_workloads=(), dagger_rounds: int = 0, dam_every: int = 3, log=sys.stderr) -> dict:
+                  eval_workloads=(), dagger_rounds: int = 0, dam_every: int = 3, dam_warmup: int = 48,
+                  log=sys.stderr) -> dict:
     """The training recipe: `scenes` trajectories, the last `held_out` kept for evaluation;
     Adam at lr for `steps` batches, then lr / 10 for a quarter as many. Each of
     `dagger_rounds` rounds then adds the same number of scenes recorded along the
@@ -386,7 +388,7 @@ def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, reco
     trajectory, used for evaluation only."""
     per_round = scenes * records * samples
     data = TrainingSet(per_round * (1 + dagger_rounds) + len(eval_workloads) * 8 * 16384)
-    ranges = generate(data, scenes, records, samples, dam_every=dam_every, log=log)
+    ranges = generate(data, scenes, records, samples, dam_every=dam_every, dam_warmup=dam_warmup, log=log)
     split = ranges[scenes - held_out][0] if held_out else len(data)
     end = len(data)
     evals = [(wl.name, record_workload(data, wl, 8, 16384, warmup_steps=4, gap_steps=4)) for wl in eval_workloads]
@@ -397,7 +399,8 @@ def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, reco
     train_ranges = [(0, split)]
     for rnd in range(dagger_rounds):
         b0 = len(data)
-        generate(data, scenes, records, samples, first=scenes * (rnd + 1), net=net, dam_every=dam_every, log=log)
+        generate(data, scenes, records, samples, first=scenes * (rnd + 1), net=net, dam_every=dam_every,
+                 dam_warmup=dam_warmup, log=log)
         train_ranges.append((b0, len(data)))
         # rows of all rounds: train on a window ending at the newest rows and covering them all
         for k, (r0, r1) in enumerate(train_ranges):
@@ -533,6 +536,7 @@ def main(argv=None) -> int:
     t.add_argument("--out", default=str(default_weights()))
     t.add_argument("--dagger", type=int, default=0, help="rounds of training on the network's own rollouts")
     t.add_argument("--dam-every", type=int, default=3, help="one training scene in this many is a dam-break column")
+    t.add_argument("--dam-warmup", type=int, default=48, help="dam columns are recorded after 0 .. this many - 1 steps")
     t.add_argument("--eval", nargs="*", default=["m1"], choices=("m1", "r4m", "s16k"),
                    help="dam breaks whose own trajectories the network is also evaluated on")
     e = sp.add_parser("compare", help="plain path against the surrogate on a dam break")
@@ -551,7 +555,7 @@ def main(argv=None) -> int:
         from . import fluid
         wls = [{"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[w]() for w in a.eval]
         rep = train_default(net, a.scenes, a.held_out, a.records, a.samples, a.steps, a.batch, a.lr, wls,
-                            a.dagger, a.dam_every)
+                            a.dagger, a.dam_every, a.dam_warmup)
         rep["seconds"] = time.perf_counter() - t0
         Path(a.out).parent.mkdir(parents=True, exist_ok=True)
         net.save(a.out)
