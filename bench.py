@@ -136,7 +136,7 @@ def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
         s.substep(p, dt, threads)
         done += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or done >= 64:
+        if el >= budget_s or done >= 160:   # at most 20 step(1/600) calls
             break
     rate_sub = w.count * done / el
     return {
