@@ -103,6 +103,7 @@ struct Launch {
     uint32_t total;               // slots (sum of capacities): neighbour-list stride
     uint32_t totalCells;
     uint32_t totalBins;           // sum of (binCapacity + 1)
+    uint32_t modes;               // OR of the scenes' SPH_MODE_* flags (picks kernel variants)
 };
 
 struct GridBuffers {

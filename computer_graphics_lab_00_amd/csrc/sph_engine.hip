@@ -373,6 +373,8 @@ sphdev::Launch Engine::launch_for(uint32_t j) const {
     L.total = totalN;
     L.totalCells = totalC;
     L.totalBins = planBins;
+    L.modes = 0u;
+    for (const SceneHost& s : sc) L.modes |= s.mode;
     return L;
 }
 

@@ -75,9 +75,11 @@ __device__ inline float eos_pressure(const StepConsts& k, float density) {
 // last batch of a bucket may read up to seven slots past it (the position arrays carry
 // kPosPad slots of padding) and ignores them.
 // With SUMS the pass also accumulates the neighbourhood sums of NeighbourSums
-// (sph_device.h) over the same candidates, in the same order.
-template <bool SUMS>
-__global__ void __launch_bounds__(kBlock)
+// (sph_device.h) over the same candidates, in the same order. DEDUPE = false is the variant
+// for launches where no scene asked for de-duplicated buckets: it keeps no bucket hashes
+// live, which leaves the reference-mode pass at 64 VGPRs (8 waves per SIMD).
+template <bool SUMS, bool DEDUPE>
+__global__ void __launch_bounds__(kBlock, (!SUMS && !DEDUPE) ? 8 : 1)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
               const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
               float4* __restrict__ sums) {
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(kBlock)
     const float px = self[0], py = self[1], pz = self[2];
     const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
     const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
-    const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
+    const bool dedupe = DEDUPE && (k.mode & SPH_MODE_DEDUPE) != 0;
     const uint32_t cap = nbrList ? k.listCap : 0u;
     float density = 0.0f;
     uint32_t listed = 0;
@@ -529,10 +531,21 @@ hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint3
                           const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
                           const NeighbourSums* sums, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
+    const bool dedupe = (L.modes & SPH_MODE_DEDUPE) != 0;
     if (sums) {
-        k_density<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, sums->sums);
+        if (dedupe)
+            k_density<true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
+                                                              sums->sums);
+        else
+            k_density<true, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
+                                                               sums->sums);
     } else {
-        k_density<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, nullptr);
+        if (dedupe)
+            k_density<false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
+                                                               nullptr);
+        else
+            k_density<false, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
+                                                                nullptr);
     }
     return hipGetLastError();
 }
