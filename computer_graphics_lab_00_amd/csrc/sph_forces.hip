@@ -77,9 +77,10 @@ __device__ inline float eos_pressure(const StepConsts& k, float density) {
 // With SUMS the pass also accumulates the neighbourhood sums of NeighbourSums
 // (sph_device.h) over the same candidates, in the same order. DEDUPE = false is the variant
 // for launches where no scene asked for de-duplicated buckets: it keeps no bucket hashes
-// live, which leaves the reference-mode pass at 64 VGPRs (8 waves per SIMD).
+// live, which leaves the reference-mode pass at 64 VGPRs (8 waves per SIMD) and the pass with
+// sums at 96 (5 waves), each with 4 VGPRs spilled, measured faster than one wave fewer.
 template <bool SUMS, bool DEDUPE>
-__global__ void __launch_bounds__(kBlock, (!SUMS && !DEDUPE) ? 8 : 1)
+__global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
               const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
               float4* __restrict__ sums) {
