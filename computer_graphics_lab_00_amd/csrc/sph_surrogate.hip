@@ -81,6 +81,11 @@ struct SphNet {
     // the prepared state of epoch `preparedEpoch`, and `iy` the network's prediction for
     // the prepared state of epoch `predictedEpoch` (0: none).
     uint64_t preparedEpoch = 0, predictedEpoch = 0;
+    // The inference buffers (sums, ix, iy, keep*, the check's) are used on the stream of the
+    // solver a call works with; a call on another solver's stream first waits for `done`,
+    // recorded at the end of the previous call on `lastStream`.
+    hipEvent_t done = nullptr;
+    hipStream_t lastStream = nullptr;
     float smoothing = 0.0f;   // XSPH velocity smoothing at the start of a macro-step (0: off)
     uint32_t maxBackoff = 0;  // after a failed check, macro-steps run plain before the next try
     uint32_t backoff = 0, skip = 0;
@@ -99,6 +104,7 @@ struct SphNet {
         if (hostCheck) hipHostFree(hostCheck);
         for (hipEvent_t e : ev)
             if (e) hipEventDestroy(e);
+        if (done) hipEventDestroy(done);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -346,6 +352,8 @@ extern "C" int sph_net_create(int32_t device, uint64_t seed, SphNet** out) {
         if (hipMalloc(p, kParams * 4) != hipSuccess) return bail(fail(SPH_ERR_OUT_OF_MEMORY, "net params"));
     if (hipMalloc(&n->wb, kParams * 2) != hipSuccess) return bail(fail(SPH_ERR_OUT_OF_MEMORY, "net params"));
     for (hipEvent_t& e : n->ev) hipEventCreate(&e);
+    if (hipEventCreateWithFlags(&n->done, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(SPH_ERR_DEVICE, "event"));
     const std::vector<float> fz = frozen_mask();
     if (hipMemcpy(n->frozen, fz.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SPH_ERR_DEVICE, "upload"));
@@ -682,6 +690,19 @@ int predict(SphNet* n, sphhost::Engine& e, uint32_t slot) {
     return SPH_OK;
 }
 
+// Orders this call's use of the inference buffers after the previous call's, when that one
+// ran on another stream.
+int net_enter(SphNet* n, hipStream_t s) {
+    if (n->lastStream && n->lastStream != s) SPH_HIP(hipStreamWaitEvent(s, n->done, 0));
+    return SPH_OK;
+}
+
+int net_leave(SphNet* n, hipStream_t s) {
+    SPH_HIP(hipEventRecord(n->done, s));
+    n->lastStream = s;
+    return SPH_OK;
+}
+
 }  // namespace
 
 // Features of every particle of the current state (index order), rows x 64 BF16 bits.
@@ -746,6 +767,7 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
     const uint32_t R = pad32(e.total_slots());
     if (int rc = net_ensure_inference(net, R)) return rc;
     hipStream_t s = e.stream;
+    if (int rc = net_enter(net, s)) return rc;
     const sphdev::NeighbourSums ns{net->sums};
     const size_t stateBytes = (size_t)e.total_slots() * sizeof(float4);
     for (uint32_t step = 0; step < steps; ++step) {
@@ -811,7 +833,7 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             st.plainSubsteps += 1;
         }
     }
-    return SPH_OK;
+    return net_leave(net, s);
 }
 
 // After a failed check the next 1, 2, 4, ... (at most maxBackoff) macro-steps run plain
@@ -851,8 +873,11 @@ extern "C" int sph_density_deviation(SphSolver* solver, SphNet* net, const SphPa
     if (!solver || !net || !p || !out) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_density_deviation: null argument");
     sphhost::Engine& e = solver->e;
     if (int rc = e.set_params(0, *p)) return rc;
+    if (int rc = e.set_device()) return rc;
+    if (int rc = net_enter(net, e.stream)) return rc;
     if (int rc = e.prepare(dtSub, nullptr)) return rc;
-    return deviation_of(net, e, out);
+    if (int rc = deviation_of(net, e, out)) return rc;
+    return net_leave(net, e.stream);
 }
 
 // C[M x N] = A[M x K] * B[K x N] for host row-major BF16 operands through the network's

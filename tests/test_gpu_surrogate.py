@@ -394,6 +394,31 @@ def test_network_trained_here_beats_gravity_alone(sph):
     data.close()
 
 
+def test_one_network_shared_by_two_solvers(sph):
+    """Rollouts of two solvers (two HIP streams) taking turns on one network, with no host
+    synchronisation between the calls, equal the same rollouts each with a network of its
+    own, bit for bit."""
+    p, pos, vel, _ = _small_scene(seed=11)
+    _, pos2, vel2, _ = _small_scene(seed=12)
+    say("one network, two solvers, interleaved rollouts")
+    inf, dt = float("inf"), 1.0 / 600.0
+    shared = _trained_net()
+    a = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    b = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    a.load_state(pos, vel)
+    b.load_state(pos2, vel2)
+    for _ in range(3):
+        surrogate.rollout(a, shared, p, dt, steps=1, threshold=inf)
+        surrogate.rollout(b, shared, p, dt, steps=1, threshold=inf)
+    ra = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    rb = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    ra.load_state(pos, vel)
+    rb.load_state(pos2, vel2)
+    surrogate.rollout(ra, _trained_net(), p, dt, steps=3, threshold=inf)
+    surrogate.rollout(rb, _trained_net(), p, dt, steps=3, threshold=inf)
+    assert _same_state(a, ra) and _same_state(b, rb)
+
+
 def _macro_gravity_only(p, pos, vel, dt_sub, max_speed, accel=None):
     """One macro-step restated: 4 substeps holding the SPH acceleration `accel` (0 when
     None) with gravity, the speed clamp and the walls, in the kernel's FP32 order."""
