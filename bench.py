@@ -209,6 +209,18 @@ def load_traffic(kernel: str):
     return None
 
 
+def load_valu_busy(kernel: str):
+    """Share of the VALU issue slots `kernel` fills, from the committed rocprofv3 SQ/GRBM
+    counter summary, or None."""
+    f = ROOT / "profiles" / "counters_dam_break_1m.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text())["kernels"][kernel]["valu_busy"]
+        except Exception:
+            return None
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -285,6 +297,10 @@ def main():
                     "achieved": b_alg * n / (substep_ms * 1e-3) / 1e9,
                     "frac": b_alg * n / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels_ms_per_substep": {k: v[0] / max(v[1], 1) for k, v in prof.items()},
+        # the kernel is instruction-issue bound, not HBM bound: the share of VALU issue slots
+        # it fills (committed rocprofv3 SQ/GRBM counters)
+        "valu_busy": load_valu_busy(dom),
+        "kernels_valu_busy": {k: load_valu_busy(k) for k in prof},
         "profiled_steps": prof_steps,
     }
 
