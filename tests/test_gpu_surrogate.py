@@ -394,6 +394,29 @@ def test_network_trained_here_beats_gravity_alone(sph):
     data.close()
 
 
+def test_rollout_with_colliders_is_the_plain_path(sph):
+    """Scenes with colliders are never handed to the network: the rollout's substeps are
+    the plain path's, bit for bit, and no macro-step is counted."""
+    say("rollout with a collider == plain")
+    col = capi.SphCollider()
+    col.shape = capi.SHAPE_SPHERE
+    col.radius = 0.3
+    col.world[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0.0, 0.5, 0.0, 1]
+    col.inverseWorld[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0.0, -0.5, 0.0, 1]
+    p = fluid.derive_fluid_params(capi.SphSettings(), fluid.translation4(0.0, 1.0, 0.0), 2048, 4096, colliders=[col])
+    assert p.colliderCount == 1
+    a = fluid.FluidSolver(2048, 4096)
+    b = fluid.FluidSolver(2048, 4096)
+    a.emit(p, 2048)
+    b.emit(p, 2048)
+    net = _trained_net()
+    st = surrogate.rollout(a, net, p, 1.0 / 600.0, steps=4, threshold=float("inf"))
+    for _ in range(4):
+        b.step(p, 1.0 / 600.0)
+    assert st.macroSteps == 0 and st.plainSubsteps == 4 * fluid.fluid_substeps(p, 1.0 / 600.0)
+    assert _same_state(a, b)
+
+
 def test_one_network_shared_by_two_solvers(sph):
     """Rollouts of two solvers (two HIP streams) taking turns on one network, with no host
     synchronisation between the calls, equal the same rollouts each with a network of its
