@@ -461,16 +461,24 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
         solver.synchronize()
         return time.perf_counter() - t0, st, series
 
+    def say(msg):
+        if log is not None:
+            print(f"[surrogate] {msg}", file=log, flush=True)
+
     run(False, True)   # warm-up: plans, graphs
+    say("warm-up done")
     t_plain, _, _ = run(False, True)
+    say(f"plain: {1e3 * t_plain / steps:.3f} ms/step")
     net.inference_time()
     t_sur, st, _ = run(True, True)
+    say(f"surrogate: {1e3 * t_sur / steps:.3f} ms/step")
     mlp_ms, mlp_rows = net.inference_time()
     if unchecked:
         # the same rollout with the check never failing: the cost of the surrogate alone
         saved = threshold
         threshold = float("inf")
         t_free, _, _ = run(True, True)
+        say(f"surrogate, check off: {1e3 * t_free / steps:.3f} ms/step")
         net.inference_time()
         result["unchecked_ms_per_step"] = 1e3 * t_free / steps
         result["unchecked_speedup"] = t_plain / t_free
