@@ -182,6 +182,30 @@ def test_s16k_trajectory_bit_exact(oracle_lib):
         assert_bit_equal(gv, ov, "trajectory velocities")
 
 
+def test_s16k_600_steps_bit_exact(oracle_lib):
+    """The whole 16,384-particle dam-break run, 600 step(1/600) calls (3,000 substeps), bit for
+    bit against the oracle, checked at steps 100, 300 and 600."""
+    say("case: s16k 600 steps")
+    w = fluid.s16k()
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    with fluid.FluidSolver(w.count, p.cellCount) as gpu:
+        gpu.load_state(pos, vel)
+        ora = oracle_lib.OracleSolver(32)
+        ora.load_state(pos, vel)
+        for step in range(1, 601):
+            gpu.step(p, 1.0 / 600.0)
+            ora.step(p, 1.0 / 600.0, THREADS)
+            if step in (100, 300, 600):
+                gp, gv = gpu.state()
+                op, ov = ora.state()
+                assert_bit_equal(gp, op, f"positions after {step} steps")
+                assert_bit_equal(gv, ov, f"velocities after {step} steps")
+                say(f"  step {step}: bit-exact")
+        # the fluid has collapsed and spread: not a trivial trajectory
+        assert gp[:, 0].max() > pos[:, 0].max() + 0.1
+
+
 def test_fluid_test_scene_60_frames_bit_exact(oracle_lib):
     """tests/fluid_test.cpp:77-96 on the GPU: containment, settling, and the same bits
     as the oracle after 60 frames of step(1/60)."""
