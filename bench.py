@@ -14,6 +14,7 @@ step on this host's cores over a bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -304,6 +305,20 @@ def main():
         "profiled_steps": prof_steps,
     }
 
+    # A renderer reads the particles back every frame (FluidSimulator::writeCpuInstances calls
+    # particles()): the same steps, each followed by the 16-byte-per-particle readback to host
+    # memory. Reported beside `value`, never as it.
+    frames = min(args.steps, 20)
+    host = np.empty((n, 4), np.float32)
+    gpu.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(frames):
+        gpu.step(p, DT)
+        capi.check(capi.load().sph_read_particles(gpu._h, host.ctypes.data_as(C.POINTER(C.c_float)), None))
+    pcie_ms = (time.perf_counter() - t1) * 1e3 / frames
+    readback = {"ms_per_step": pcie_ms, "particle_steps_per_s": n / (pcie_ms * 1e-3), "frames": frames,
+                "bytes_per_frame": 16 * n, "note": "step(1/600) + sph_read_particles into pageable host memory"}
+
     sur = None
     if not args.no_surrogate:
         print("[bench] surrogate leg", file=sys.stderr, flush=True)
@@ -334,6 +349,7 @@ def main():
                    "particle_substeps_per_s": value * substeps},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "with_readback": readback,
         "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
