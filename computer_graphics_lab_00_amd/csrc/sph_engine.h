@@ -169,14 +169,20 @@ struct SphSolver {
     SphSurfaceVertex* surfVerts = nullptr;
     uint32_t surfVertCap = 0;
     uint32_t* surfCounters = nullptr;
+    // device staging of the renderer export, grown on demand
+    SphGpuInstance* instStage = nullptr;
+    SphTlasInstance* tlasStage = nullptr;
+    uint32_t instCap = 0, tlasCap = 0;
 
     ~SphSolver() {
-        if (surfField || surfVerts || surfCounters) {
+        if (surfField || surfVerts || surfCounters || instStage || tlasStage) {
             hipSetDevice(e.device);
             hipDeviceSynchronize();
             hipFree(surfField);
             hipFree(surfVerts);
             hipFree(surfCounters);
+            hipFree(instStage);
+            hipFree(tlasStage);
         }
     }
 };

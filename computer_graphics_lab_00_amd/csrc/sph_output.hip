@@ -125,15 +125,25 @@ extern "C" int sph_write_instances(SphSolver* solver, const SphParams* p, const 
     a.pad = 0u;
     a.sphereBlas = cfg->sphereBlas;
     a.surfaceBlas = cfg->surfaceBlas;
-    SphGpuInstance* di = nullptr;
-    SphTlasInstance* dt = nullptr;
-    SPH_HIP(hipMalloc(&di, (size_t)n * sizeof(SphGpuInstance)));
-    hipError_t he = a.tlasMode ? hipMalloc(&dt, (size_t)n * sizeof(SphTlasInstance)) : hipSuccess;
-    if (he == hipSuccess) {
-        k_instances<<<(n + kThreads - 1) / kThreads, kThreads, 0, e.stream>>>(e.state_pos(), e.state_ids(),
-                                                                              e.previous_pos(), a, di, dt);
-        he = hipGetLastError();
+    if (solver->instCap < n) {
+        hipFree(solver->instStage);
+        solver->instStage = nullptr;
+        solver->instCap = 0;
+        SPH_HIP(hipMalloc(&solver->instStage, (size_t)n * sizeof(SphGpuInstance)));
+        solver->instCap = n;
     }
+    if (a.tlasMode && solver->tlasCap < n) {
+        hipFree(solver->tlasStage);
+        solver->tlasStage = nullptr;
+        solver->tlasCap = 0;
+        SPH_HIP(hipMalloc(&solver->tlasStage, (size_t)n * sizeof(SphTlasInstance)));
+        solver->tlasCap = n;
+    }
+    SphGpuInstance* di = solver->instStage;
+    SphTlasInstance* dt = a.tlasMode ? solver->tlasStage : nullptr;
+    k_instances<<<(n + kThreads - 1) / kThreads, kThreads, 0, e.stream>>>(e.state_pos(), e.state_ids(),
+                                                                          e.previous_pos(), a, di, dt);
+    hipError_t he = hipGetLastError();
     if (he == hipSuccess)
         he = hipMemcpyAsync(instances + cfg->instanceBase, di, (size_t)n * sizeof(SphGpuInstance),
                             hipMemcpyDeviceToHost, e.stream);
@@ -141,8 +151,6 @@ extern "C" int sph_write_instances(SphSolver* solver, const SphParams* p, const 
         he = hipMemcpyAsync(tlas + cfg->tlasBase, dt, (size_t)n * sizeof(SphTlasInstance), hipMemcpyDeviceToHost,
                             e.stream);
     if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
-    hipFree(di);
-    hipFree(dt);
     if (he != hipSuccess) return hip_fail(he, "sph_write_instances");
     return e.keep_rendered(0);
 }
