@@ -38,7 +38,7 @@ MLP_FLOP_PER_ROW = 2 * (64 * 128 + 128 * 128 + 128 * 128 + 128 * 32)
 ROLLOUT_MLP_FLOP_PER_ROW = 2 * (32 * 128 + 128 * 128 + 128 * 128 + 128 * 32)
 MACRO = 4
 DEFAULT_THRESHOLD = 0.01
-DEFAULT_SMOOTHING = 0.6   # XSPH smoothing of the measured configuration
+DEFAULT_SMOOTHING = 0.95  # XSPH smoothing of the measured configuration (swept 0.3 .. 1.0)
 DEFAULT_BACKOFF = 8       # plain macro-steps after repeated failed checks, at most
 _FP = C.POINTER(C.c_float)
 _HP = C.POINTER(C.c_uint16)
