@@ -278,7 +278,7 @@ def _trained_net(params=None):
     net.load(surrogate.default_weights())
     if params is not None:
         net.params = params
-    net.set_smoothing(0.6)
+    net.set_smoothing(surrogate.DEFAULT_SMOOTHING)
     return net
 
 
@@ -318,7 +318,7 @@ def test_rollout_is_never_served_a_stale_prediction(sph):
     # new weights between two calls
     w = net.params * F32(0.5)
     net.params = w
-    net.set_smoothing(0.6)
+    net.set_smoothing(surrogate.DEFAULT_SMOOTHING)
     surrogate.rollout(a, net, p, dt, steps=1, threshold=inf)
     other = _trained_net(w)
     surrogate.rollout(c, other, p, dt, steps=1, threshold=inf)
@@ -346,9 +346,9 @@ def test_trained_network_gains_over_the_no_network_baselines(sph):
     data.close()
 
     # Two dam-break columns of the family outside the training set (one per kernel radius),
-    # from the state after their warm-up: 10 steps of the rollout as configured (XSPH 0.6,
+    # from the state after their warm-up: 10 steps of the rollout as configured (XSPH default,
     # check off) against the same rollout with a zero network.
-    net.set_smoothing(0.6)
+    net.set_smoothing(surrogate.DEFAULT_SMOOTHING)
     zero = surrogate.SurrogateNet(seed=3)
     zero.params = np.zeros(zero.param_count, F32)
     dt, inf = 1.0 / 600.0, float("inf")
