@@ -164,7 +164,10 @@ __device__ __forceinline__ Where locate(const Launch& L) {
 //   sums[4 * total + t] = (sum s^2 (w x e),           0)
 constexpr uint32_t kSumRows = 5;
 struct NeighbourSums {
-    float4* sums;   // kSumRows x total
+    float4* sums;                  // kSumRows x total
+    float2* deviation = nullptr;   // when set: per workgroup of the pass, the sums over its particles
+                                   // of max(rho / rho0 - 1, 0) and |rho / rho0 - 1| (a tree over the
+                                   // workgroup's lanes in LDS), the density check's first stage
 };
 
 // launchers (sph_grid.hip); all batched launches use L.blocks workgroups

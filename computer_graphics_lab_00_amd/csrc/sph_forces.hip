@@ -122,15 +122,17 @@ template <bool SUMS, bool DEDUPE>
 __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
               const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
-              float4* __restrict__ sums) {
+              float4* __restrict__ sums, float2* __restrict__ deviation) {
     const Where w = locate(L);
-    if (!w.valid || !w.d->active) return;
+    const bool live = w.valid && w.d->active;
+    // With a deviation output every lane stays to the end, where the workgroup reduces it.
+    if (!live && !(SUMS && deviation)) return;
     const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
     const uint32_t t = w.slot;
     const uint32_t cellBase = w.d->cellOffset;
     // Only xyz of any slot is read here; .w of this lane's own slot is written below
     // (the reference writes positions[i].w in place the same way, fluid_sph.cpp:197-199).
-    const float* self = reinterpret_cast<const float*>(pos + t);
+    const float* self = reinterpret_cast<const float*>(pos + (live ? t : w.d->offset));
     const float px = self[0], py = self[1], pz = self[2];
     const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
     const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
 #pragma unroll
         for (uint32_t o = 0; o < 9; ++o) {
             if (dedupe && seen_before(cx, cy, cz, (uint32_t)(dz + 1) * 9u + o, bk[o], k.cellMask)) continue;
-            uint32_t cnt = cn[o];
+            uint32_t cnt = live ? cn[o] : 0u;
             if (cnt == 0) continue;
             const uint32_t lo = st[o];
             if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
@@ -234,6 +236,25 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
     }
     if (SUMS) sum_pending();
     density = density < 1e-6f ? 1e-6f : density;
+    if (SUMS && deviation) {
+        // the density check's workgroup sums, in the same tree as a separate reduction pass
+        const float dv = density / k.restDensity - 1.0f;
+        __syncthreads();   // every lane is done with its ring
+        float* sa = reinterpret_cast<float*>(&ring[0][0]);
+        float* sb = sa + kBlock;
+        sa[threadIdx.x] = live ? (dv > 0.0f ? dv : 0.0f) : 0.0f;
+        sb[threadIdx.x] = live ? (dv < 0.0f ? -dv : dv) : 0.0f;
+        __syncthreads();
+        for (uint32_t h = kBlock / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h) {
+                sa[threadIdx.x] += sa[threadIdx.x + h];
+                sb[threadIdx.x] += sb[threadIdx.x + h];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) deviation[blockIdx.x] = make_float2(sa[0], sb[0]);
+        if (!live) return;
+    }
     reinterpret_cast<float*>(pos + t)[3] = density;
     reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
     if (nbrCount) {
@@ -582,17 +603,17 @@ hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint3
     if (sums) {
         if (dedupe)
             k_density<true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nullptr, nullptr,
-                                                              sums->sums);
+                                                              sums->sums, sums->deviation);
         else
             k_density<true, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nullptr, nullptr,
-                                                               sums->sums);
+                                                               sums->sums, sums->deviation);
     } else {
         if (dedupe)
             k_density<false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
-                                                               nullptr);
+                                                               nullptr, nullptr);
         else
             k_density<false, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
-                                                                nullptr);
+                                                                nullptr, nullptr);
     }
     return hipGetLastError();
 }

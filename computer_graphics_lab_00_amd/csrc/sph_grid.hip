@@ -13,13 +13,14 @@
 // sit next to each other in HBM, which is what the 27-bucket gathers read.
 // All scenes of a batch go through each step in one launch (workgroups find their scene
 // in a table); the single solver is a batch of one.
-//   1. clear:   counts = 0, rep = ~0, bin counts = 0; each scene's filler bin = its
-//               unused capacity, so one global scan aligns every scene to its slots
+//   1. clear:   counts = 0, rep = ~0, bin counts = 0
 //   2. keys:    key = spatialHash(spatialCell(p, h)); counts[key] += 1; rep[key] min= bin
 //               (wave-level run aggregation: the state is already spatially sorted, so
-//               neighbouring lanes mostly share a bucket)
+//               neighbouring lanes mostly share a bucket); each scene's filler bin = its
+//               unused capacity, so one global scan aligns every scene to its slots
 //   3. binslot: bin = rep[key]; a provisional slot in the bin by atomic
-//   4. scan:    exclusive scan of binCount -> binStart (three passes, LDS + shuffles)
+//   4. scan:    exclusive scan of binCount -> binStart (two passes: tiles scanned in LDS
+//               with shuffles, then each tile adds the sum of the tile totals before it)
 //   5. scatter: (key, id, source) lands in its bin's range
 //   6. rank:    final slot = bin start + rank of (key, id) among the bin's members; the
 //               particle's pos / vel / id are copied there, and the first member of
@@ -100,9 +101,17 @@ __global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ ids, uin
     if (i < n) ids[i] = i;
 }
 
+// Also sets each scene's filler bin (workgroup 0; the bin counts were cleared before and
+// nothing else touches them until k_bin_slots).
 __global__ void __launch_bounds__(kBlock)
     k_keys(Launch L, const float4* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ counts,
-           uint32_t* __restrict__ rep) {
+           uint32_t* __restrict__ rep, uint32_t* __restrict__ binCount) {
+    if (blockIdx.x == 0) {
+        for (uint32_t s = threadIdx.x; s < L.numScenes; s += kBlock) {
+            const SceneDesc& d = L.scenes[s];
+            binCount[d.binOffset + d.binCapacity] = d.capacity - d.k.n;
+        }
+    }
     const Where w = locate(L);
     const StepConsts k = w.d->k;   // by value: the constants stay in scalar registers
     const uint32_t lane = lane_id();
@@ -208,37 +217,21 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
 }
 
-// Pass 2: one block scans the tile totals (exclusive), serially per thread chunk.
-__global__ void __launch_bounds__(1024) k_scan_totals(uint32_t* __restrict__ sums, uint32_t tiles) {
-    __shared__ uint32_t waveSums[16];
-    const uint32_t per = (tiles + 1023u) / 1024u;
-    const uint32_t begin = threadIdx.x * per;
-    uint32_t local = 0;
-    for (uint32_t e = 0; e < per; ++e) {
-        uint32_t idx = begin + e;
-        if (idx < tiles) local += sums[idx];
-    }
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t inc = wave_inclusive_scan(local);
-    if (lane == 63) waveSums[wave] = inc;
-    __syncthreads();
-    uint32_t off = 0;
-    for (uint32_t w = 0; w < wave; ++w) off += waveSums[w];
-    uint32_t run = off + inc - local;
-    for (uint32_t e = 0; e < per; ++e) {
-        uint32_t idx = begin + e;
-        if (idx < tiles) {
-            uint32_t x = sums[idx];
-            sums[idx] = run;
-            run += x;
-        }
-    }
-}
-
-// Pass 3: add each tile's offset; the element after the last one gets the total.
+// Pass 2: each tile adds the sum of the tile totals before it (every workgroup reduces that
+// prefix of the totals itself: a few thousand L2-resident words); the element after the last
+// one gets the total.
 __global__ void __launch_bounds__(kBlock)
     k_scan_add(uint32_t* __restrict__ data, uint32_t count, uint32_t total, const uint32_t* __restrict__ sums) {
-    const uint32_t add = sums[blockIdx.x];
+    __shared__ uint32_t waveSums[kBlock / 64];
+    uint32_t local = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kBlock) local += sums[i];
+#pragma unroll
+    for (uint32_t d = 32; d > 0; d >>= 1) local += __shfl_down(local, d, 64);
+    if ((threadIdx.x & 63u) == 0u) waveSums[threadIdx.x >> 6] = local;
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; ++w) add += waveSums[w];
     const uint32_t base = blockIdx.x * kScanTile;
     for (uint32_t e = threadIdx.x; e < kScanTile; e += kBlock) {
         uint32_t idx = base + e;
@@ -355,12 +348,15 @@ hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t 
     uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
     k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins);
-    k_grid_fillers<<<1, kBlock, 0, s>>>(L, b.binStart);
     return hipGetLastError();
 }
 
 hipError_t launch_keys(const Launch& L, const GridBuffers& b, hipStream_t s) {
-    k_keys<<<L.blocks, kBlock, 0, s>>>(L, b.pos, b.keys, b.counts, b.rep);
+    if (L.blocks == 0) {   // no particle anywhere: only the filler bins
+        k_grid_fillers<<<1, kBlock, 0, s>>>(L, b.binStart);
+        return hipGetLastError();
+    }
+    k_keys<<<L.blocks, kBlock, 0, s>>>(L, b.pos, b.keys, b.counts, b.rep, b.binStart);
     return hipGetLastError();
 }
 
@@ -372,7 +368,6 @@ hipError_t launch_bin_slots(const Launch& L, const GridBuffers& b, hipStream_t s
 hipError_t launch_scan(uint32_t* data, uint32_t count, uint32_t total, uint32_t* blockSums, hipStream_t s) {
     uint32_t tiles = (count + kScanTile - 1) / kScanTile;
     k_scan_tiles<<<tiles, kBlock, 0, s>>>(data, count, blockSums);
-    k_scan_totals<<<1, 1024, 0, s>>>(blockSums, tiles);
     k_scan_add<<<tiles, kBlock, 0, s>>>(data, count, total, blockSums);
     return hipGetLastError();
 }

@@ -67,6 +67,10 @@ hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, co
                                   float4* velKeep, hipStream_t s);
 hipError_t launch_deviation(const sphdev::Launch& L, const float4* pos, float2* partial, double* out, double invCount,
                             hipStream_t s);
+// The second stage alone, over workgroup sums a density pass with NeighbourSums::deviation
+// has written.
+hipError_t launch_deviation_total(const sphdev::Launch& L, const float2* partial, double* out, double invCount,
+                                  hipStream_t s);
 hipError_t launch_gather_rows(const uint16_t* x, const uint32_t* ids, const uint32_t* slots, uint32_t count,
                               uint16_t* dst, uint32_t* dstIds, hipStream_t s);
 hipError_t launch_accumulate_accel(const float4* force, const float4* pos, const uint32_t* ids, uint32_t n,

@@ -77,6 +77,10 @@ struct SphNet {
     // 0..3: begin/end of the network pass of prediction slots 0 and 1; 4: density check copied
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t predictedSlot = 0;   // event pair of the prediction held in iy
+    // One network pass in eight is timed (an event pair on the stream costs a few
+    // microseconds of idle GPU); timed[slot]: the pass in that slot was.
+    uint32_t predictions = 0;
+    bool timed[2] = {false, false};
     // Engine epochs (unique in the process): the solver's sorted buffers and `sums` hold
     // the prepared state of epoch `preparedEpoch`, and `iy` the network's prediction for
     // the prepared state of epoch `predictedEpoch` (0: none).
@@ -636,7 +640,9 @@ namespace {
 
 // Enqueues the density statistics of the prepared state (reduced on the device into
 // pinned host memory), then event 4.
-int launch_check(SphNet* n, sphhost::Engine& e) {
+// The check's buffers: mapped host memory for its two means and one workgroup sum pair per
+// workgroup of the solver's launches.
+int ensure_check(SphNet* n, sphhost::Engine& e) {
     const sphdev::Launch L = e.plan_launch();
     if (!n->hostCheck) {
         SPH_HIP(hipHostMalloc(&n->hostCheck, 2 * sizeof(double), hipHostMallocMapped));
@@ -650,11 +656,22 @@ int launch_check(SphNet* n, sphhost::Engine& e) {
         SPH_HIP(hipMalloc(&n->devPartial, (size_t)L.blocks * sizeof(float2)));
         n->devPartialCap = L.blocks;
     }
+    return SPH_OK;
+}
+
+// The density check of the prepared state; `summed`: the prepare's density pass already wrote
+// the workgroup sums into devPartial.
+int launch_check(SphNet* n, sphhost::Engine& e, bool summed) {
+    const sphdev::Launch L = e.plan_launch();
+    if (int rc = ensure_check(n, e)) return rc;
     n->hostCheck[0] = 0.0;
     n->hostCheck[1] = 0.0;
     const uint32_t count = e.sc[0].n;
-    SPH_HIP(sphsur::launch_deviation(L, e.sorted_pos(), n->devPartial, n->hostCheckDev,
-                                     count ? 1.0 / (double)count : 0.0, e.stream));
+    const double inv = count ? 1.0 / (double)count : 0.0;
+    if (summed)
+        SPH_HIP(sphsur::launch_deviation_total(L, n->devPartial, n->hostCheckDev, inv, e.stream));
+    else
+        SPH_HIP(sphsur::launch_deviation(L, e.sorted_pos(), n->devPartial, n->hostCheckDev, inv, e.stream));
     SPH_HIP(hipEventRecord(n->ev[4], e.stream));
     return SPH_OK;
 }
@@ -670,7 +687,7 @@ int finish_check(SphNet* n, sphhost::Engine& e, double out[2]) {
 }
 
 int deviation_of(SphNet* n, sphhost::Engine& e, double out[2]) {
-    if (int rc = launch_check(n, e)) return rc;
+    if (int rc = launch_check(n, e, false)) return rc;
     return finish_check(n, e, out);
 }
 
@@ -680,11 +697,13 @@ int predict(SphNet* n, sphhost::Engine& e, uint32_t slot) {
     const sphdev::Launch L = e.plan_launch();
     const SphParams& p = e.sc[0].params;
     hipStream_t s = e.stream;
-    SPH_HIP(hipEventRecord(n->ev[2 * slot], s));
+    const bool timed = (n->predictions++ & 7u) == 7u;   // not the first pass of a run: it warms up
+    if (timed) SPH_HIP(hipEventRecord(n->ev[2 * slot], s));
     SPH_HIP(sphmlp::launch_mlp_fused_sums(e.sorted_pos(), n->sums, L.total, e.sc[0].n, p.restDensity,
                                           p.smoothingRadius, 1.0f / sphsur::kVelocityScale, n->wb, n->w,
                                           reinterpret_cast<float4*>(n->iy), 0, s));
-    SPH_HIP(hipEventRecord(n->ev[2 * slot + 1], s));
+    if (timed) SPH_HIP(hipEventRecord(n->ev[2 * slot + 1], s));
+    n->timed[slot] = timed;
     n->predictedEpoch = e.epoch();
     n->predictedSlot = slot;
     return SPH_OK;
@@ -768,7 +787,9 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
     if (int rc = net_ensure_inference(net, R)) return rc;
     hipStream_t s = e.stream;
     if (int rc = net_enter(net, s)) return rc;
-    const sphdev::NeighbourSums ns{net->sums};
+    if (int rc = ensure_check(net, e)) return rc;
+    // every prepare here also writes the density check's workgroup sums
+    const sphdev::NeighbourSums ns{net->sums, net->devPartial};
     const size_t stateBytes = (size_t)e.total_slots() * sizeof(float4);
     for (uint32_t step = 0; step < steps; ++step) {
         for (uint32_t m = 0; m < macros; ++m) {
@@ -795,14 +816,14 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             e.commit_sorted();
             if (int rc = e.prepare(dtSub, &ns)) return rc;
             net->preparedEpoch = e.epoch();
-            if (int rc = launch_check(net, e)) return rc;
+            if (int rc = launch_check(net, e, true)) return rc;
             // the next macro-step's prediction, queued before the host waits for the check
             if (m + 1 < macros || rest == 0)
                 if (int rc = predict(net, e, 1u - used)) return rc;
             double dev[2];
             if (int rc = finish_check(net, e, dev)) return rc;
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, net->ev[2 * used], net->ev[2 * used + 1]) == hipSuccess) {
+            if (net->timed[used] && hipEventElapsedTime(&ms, net->ev[2 * used], net->ev[2 * used + 1]) == hipSuccess) {
                 net->mlpMs += ms;
                 net->mlpRows += n;
             }
@@ -855,8 +876,8 @@ extern "C" int sph_net_set_smoothing(SphNet* n, float smoothing) {
     return SPH_OK;
 }
 
-// Time the network's forward passes took in sph_rollout since the last call (ms) and the
-// rows they processed; for the MFMA-utilisation figure.
+// Time the timed network passes (one in eight) took in sph_rollout since the last call (ms)
+// and the rows they processed; for the MFMA-utilisation figure.
 extern "C" int sph_net_inference_time(SphNet* n, double* ms, uint64_t* rows) {
     if (!n) return fail(SPH_ERR_INVALID_ARGUMENT, "null net");
     if (ms) *ms = n->mlpMs;

@@ -302,6 +302,13 @@ hipError_t launch_deviation(const Launch& L, const float4* pos, float2* partial,
     return hipGetLastError();
 }
 
+hipError_t launch_deviation_total(const Launch& L, const float2* partial, double* out, double invCount,
+                                  hipStream_t s) {
+    if (L.blocks == 0) return hipSuccess;
+    k_deviation_total<<<1, kBlock, 0, s>>>(partial, L.blocks, invCount, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_rows(const uint16_t* x, const uint32_t* ids, const uint32_t* slots, uint32_t count,
                               uint16_t* dst, uint32_t* dstIds, hipStream_t s) {
     if (count == 0) return hipSuccess;

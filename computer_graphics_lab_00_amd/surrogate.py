@@ -506,7 +506,8 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     if mlp_ms > 0:
         flop = ROLLOUT_MLP_FLOP_PER_ROW * mlp_rows
         result["mlp_tflops"] = flop / (mlp_ms * 1e-3) / 1e12
-        result["mlp_ms_per_macro_step"] = mlp_ms / max(1, st.macroSteps)
+        # one network pass per macro-step; the rollout times one pass in eight
+        result["mlp_ms_per_macro_step"] = mlp_ms * n / mlp_rows
     if log is not None:
         print(f"[surrogate] timed: plain {result['plain_ms_per_step']:.3f} ms/step, surrogate "
               f"{result['surrogate_ms_per_step']:.3f} ms/step, fallbacks {st.fallbacks}/{st.macroSteps}",

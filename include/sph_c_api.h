@@ -454,8 +454,8 @@ int sph_net_set_smoothing(SphNet* net, float smoothing);
  * maxBackoff) macro-steps run plain before the network is tried again; a passed check
  * resets it. 0 (the default) tries the network on every macro-step. */
 int sph_net_set_backoff(SphNet* net, uint32_t maxBackoff);
-/* Milliseconds spent in the network's forward passes inside sph_rollout since the last
- * call, and the rows they covered; resets both. */
+/* Milliseconds spent in the timed network forward passes inside sph_rollout since the last
+ * call (one pass in eight is timed), and the rows they covered; resets both. */
 int sph_net_inference_time(SphNet* net, double* ms, uint64_t* rows);
 /* out[0] = mean max(rho / rho0 - 1, 0), out[1] = mean |rho / rho0 - 1| of the current
  * state, from a density pass of a substep of length dtSub. */
