@@ -168,19 +168,20 @@ struct SphSolver {
     size_t surfFieldBytes = 0;
     SphSurfaceVertex* surfVerts = nullptr;
     uint32_t surfVertCap = 0;
-    uint32_t* surfCounters = nullptr;
+    uint32_t* surfMarch = nullptr;   // the marching passes' counters, offsets and scan sums
+    size_t surfMarchWords = 0;
     // device staging of the renderer export, grown on demand
     SphGpuInstance* instStage = nullptr;
     SphTlasInstance* tlasStage = nullptr;
     uint32_t instCap = 0, tlasCap = 0;
 
     ~SphSolver() {
-        if (surfField || surfVerts || surfCounters || instStage || tlasStage) {
+        if (surfField || surfVerts || surfMarch || instStage || tlasStage) {
             hipSetDevice(e.device);
             hipDeviceSynchronize();
             hipFree(surfField);
             hipFree(surfVerts);
-            hipFree(surfCounters);
+            hipFree(surfMarch);
             hipFree(instStage);
             hipFree(tlasStage);
         }

@@ -8,9 +8,11 @@
 // so each bucket is one contiguous run in ascending particle index (the reference's
 // FluidGrid order, 32 per bucket), and one lane per sample point walks the 27 buckets
 // around it with the reference's de-duplication, in its order and arithmetic: the field
-// is the reference's bit for bit. Marching runs one lane per cell; vertices go to slots
-// taken with an atomic counter, so their order (not their values) depends on timing, as
-// in the reference's threaded loop.
+// is the reference's bit for bit. Marching takes two passes over the cells: one counts each
+// workgroup's vertices, a scan turns the counts into offsets, and the second hands each
+// non-empty workgroup's vertices to its lanes one vertex per lane (a surface cell has up to
+// 15 vertices, most cells none), so the vertices come out in the reference's serial order
+// (cells x fastest, each cell's in table order) and the buffer is the serial loop's.
 #include <math.h>
 #include <string.h>
 
@@ -28,6 +30,7 @@ constexpr uint32_t kThreads = 256;
 constexpr uint32_t kMaxResolution = 1024;
 
 __constant__ int8_t c_table[256 * 16];
+__constant__ uint8_t c_nverts[256];   // vertices of each cube case (its table row up to the first -1)
 
 __device__ __constant__ const int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                                          {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
@@ -267,72 +270,128 @@ __device__ inline uint32_t pack_unit(float nx, float ny, float nz) {
     return q(u) | (q(v) << 16);
 }
 
-// marchFluidField (marching_cubes.cpp:393-487), one lane per cell. Each wave takes the slots
-// of all its cells' vertices with one atomic (a scan of the lanes' vertex counts hands each
-// lane its range), so ranges stay contiguous and disjoint in the order the waves got them.
-__global__ void __launch_bounds__(kThreads)
-    k_march(const float* __restrict__ f, MarchArgs m, SphSurfaceVertex* __restrict__ out,
-            uint32_t* __restrict__ counters) {
-    const uint32_t r = m.resolution, s = r + 1;
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u;
-    const bool inside = i < r * r * r;
-    const int32_t bx = (int32_t)(i % r), by = (int32_t)((i / r) % r), bz = (int32_t)(i / (r * r));
-    float val[8];
+// Cube case of cell (bx, by, bz) and its corner values (marching_cubes.cpp:400-410).
+__device__ inline uint32_t cube_case(const float* __restrict__ f, uint32_t s, int32_t bx, int32_t by, int32_t bz,
+                                     float iso, float val[8]) {
     uint32_t code = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        val[c] = inside ? field_at(f, s, bx + c_corner[c][0], by + c_corner[c][1], bz + c_corner[c][2]) : 0.0f;
-        if (val[c] >= m.iso) code |= 1u << c;
+        val[c] = field_at(f, s, bx + c_corner[c][0], by + c_corner[c][1], bz + c_corner[c][2]);
+        if (val[c] >= iso) code |= 1u << c;
     }
-    if (!inside) code = 0u;
-    const int8_t* entry = c_table + code * 16u;
+    return code;
+}
+
+// Sum of one value over the workgroup (wave shuffles, then the four wave sums).
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t* scratch4) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    if ((threadIdx.x & 63u) == 0u) scratch4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint32_t t = scratch4[0] + scratch4[1] + scratch4[2] + scratch4[3];
+    __syncthreads();
+    return t;
+}
+
+// Pass 1: the vertex count of each workgroup of kThreads cells (cells x fastest).
+__global__ void __launch_bounds__(kThreads)
+    k_march_count(const float* __restrict__ f, MarchArgs m, uint32_t* __restrict__ blockCount) {
+    __shared__ uint32_t waves[kThreads / 64];
+    const uint32_t r = m.resolution, s = r + 1;
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     uint32_t nv = 0;
-    while (nv < 16u && entry[nv] >= 0) ++nv;
-    // wave-wide inclusive scan of nv (every lane of the wave takes part)
+    if (i < r * r * r) {
+        float val[8];
+        nv = c_nverts[cube_case(f, s, (int32_t)(i % r), (int32_t)((i / r) % r), (int32_t)(i / (r * r)), m.iso, val)];
+    }
+    const uint32_t total = block_sum(nv, waves);
+    if (threadIdx.x == 0) blockCount[blockIdx.x] = total;
+}
+
+// Pass 2 (marchFluidField, marching_cubes.cpp:411-484): the workgroup's vertices, one per lane,
+// at the workgroup's offset from the scan. A cell whose vertices would pass `capacity` writes
+// none and records its first slot; the serial loop stops there (:467-472), and every later
+// cell starts further on, so the written count is the smallest such slot.
+__global__ void __launch_bounds__(kThreads)
+    k_march_emit(const float* __restrict__ f, MarchArgs m, const uint32_t* __restrict__ blockStart, uint32_t blocks,
+                 SphSurfaceVertex* __restrict__ out, uint32_t* __restrict__ counters) {
+    __shared__ uint32_t cellOff[kThreads + 1];
+    __shared__ uint8_t cellCode[kThreads];
+    __shared__ uint32_t waves[kThreads / 64];
+    const uint32_t b = blockIdx.x;
+    const uint32_t base = blockStart[b];
+    const bool last = b + 1u == blocks;
+    if (!last && blockStart[b + 1u] == base) return;   // no vertex in this workgroup (all lanes agree)
+    const uint32_t r = m.resolution, s = r + 1;
+    const uint32_t i = b * kThreads + threadIdx.x;
+    uint32_t nv = 0, code = 0;
+    if (i < r * r * r) {
+        float val[8];
+        code = cube_case(f, s, (int32_t)(i % r), (int32_t)((i / r) % r), (int32_t)(i / (r * r)), m.iso, val);
+        nv = c_nverts[code];
+    }
+    // exclusive scan of the cells' vertex counts in the workgroup
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t incl = nv;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t up = __shfl_up(incl, d, 64);
         if (lane >= (uint32_t)d) incl += up;
     }
-    const uint32_t waveTotal = __shfl(incl, 63, 64);
-    uint32_t base = 0u;
-    if (lane == 63u && waveTotal) base = atomicAdd(&counters[0], waveTotal);
-    base = __shfl(base, 63, 64);
-    if (nv == 0u) return;
-    const uint32_t slot = base + incl - nv;
-    if (slot + nv > m.capacity) {
-        atomicMin(&counters[1], slot);   // the first slot that did not fit (:467-472)
-        return;
+    if (lane == 63u) waves[wave] = incl;
+    cellCode[threadIdx.x] = (uint8_t)code;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kThreads / 64; ++w) {
+        before += w < wave ? waves[w] : 0u;
+        total += waves[w];
     }
-    for (uint32_t v = 0; v < nv; ++v) {
-        const uint32_t e = (uint32_t)entry[v];
-        const uint32_t a = c_edge[e][0], b = c_edge[e][1];
-        const float va = val[a], vb = val[b];
+    cellOff[threadIdx.x] = before + incl - nv;
+    if (threadIdx.x == 0) cellOff[kThreads] = total;
+    if (nv && base + before + incl > m.capacity) atomicMin(&counters[1], base + before + incl - nv);
+    if (last && threadIdx.x == 0) counters[0] = base + total;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < total; k += kThreads) {
+        // the cell holding vertex k: the last cell whose offset is <= k (empty cells share
+        // their successor's offset, so that is the one with vertices)
+        uint32_t lo = 0, hi = kThreads;   // cellOff[lo] <= k < cellOff[hi]
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cellOff[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t c = lo, cc = cellCode[c];
+        const uint32_t first = base + cellOff[c];
+        if (first + c_nverts[cc] > m.capacity) continue;   // the cell does not fit: none of it
+        const uint32_t ci = b * kThreads + c;
+        const int32_t bx = (int32_t)(ci % r), by = (int32_t)((ci / r) % r), bz = (int32_t)(ci / (r * r));
+        const uint32_t e = (uint32_t)c_table[cc * 16u + (k - cellOff[c])];
+        const uint32_t a = c_edge[e][0], bb = c_edge[e][1];
+        const int32_t ca[3] = {bx + c_corner[a][0], by + c_corner[a][1], bz + c_corner[a][2]};
+        const int32_t cb[3] = {bx + c_corner[bb][0], by + c_corner[bb][1], bz + c_corner[bb][2]};
+        const float va = field_at(f, s, ca[0], ca[1], ca[2]), vb = field_at(f, s, cb[0], cb[1], cb[2]);
         float t = fabsf(vb - va) > 1.0e-8f ? (m.iso - va) / (vb - va) : 0.5f;
         t = t < 0.0f ? 0.0f : (1.0f < t ? 1.0f : t);
-        const int32_t ca[3] = {bx + c_corner[a][0], by + c_corner[a][1], bz + c_corner[a][2]};
-        const int32_t cb[3] = {bx + c_corner[b][0], by + c_corner[b][1], bz + c_corner[b][2]};
-        float p[3], ga[3], gb[3];
+        float pnt[3], ga[3], gb[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float pa = m.origin[k] + (float)ca[k] * m.cellSize[k];
-            const float pb = m.origin[k] + (float)cb[k] * m.cellSize[k];
-            p[k] = pa + (pb - pa) * t;
+        for (int q = 0; q < 3; ++q) {
+            const float pa = m.origin[q] + (float)ca[q] * m.cellSize[q];
+            const float pb = m.origin[q] + (float)cb[q] * m.cellSize[q];
+            pnt[q] = pa + (pb - pa) * t;
         }
         // central differences over the cell size (:446-455)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int32_t ux = k == 0, uy = k == 1, uz = k == 2;
-            ga[k] = (field_at(f, s, ca[0] + ux, ca[1] + uy, ca[2] + uz) -
-                     field_at(f, s, ca[0] - ux, ca[1] - uy, ca[2] - uz)) / m.cellSize[k];
-            gb[k] = (field_at(f, s, cb[0] + ux, cb[1] + uy, cb[2] + uz) -
-                     field_at(f, s, cb[0] - ux, cb[1] - uy, cb[2] - uz)) / m.cellSize[k];
+        for (int q = 0; q < 3; ++q) {
+            const int32_t ux = q == 0, uy = q == 1, uz = q == 2;
+            ga[q] = (field_at(f, s, ca[0] + ux, ca[1] + uy, ca[2] + uz) -
+                     field_at(f, s, ca[0] - ux, ca[1] - uy, ca[2] - uz)) / m.cellSize[q];
+            gb[q] = (field_at(f, s, cb[0] + ux, cb[1] + uy, cb[2] + uz) -
+                     field_at(f, s, cb[0] - ux, cb[1] - uy, cb[2] - uz)) / m.cellSize[q];
         }
         float n[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) n[k] = -(ga[k] + (gb[k] - ga[k]) * t);   // outward: against the gradient
+        for (int q = 0; q < 3; ++q) n[q] = -(ga[q] + (gb[q] - ga[q]) * t);   // outward: against the gradient
         const float dd = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2];
         if (!(sqrtf(dd) > 1.0e-12f)) {
             n[0] = 0.0f;
@@ -342,19 +401,39 @@ __global__ void __launch_bounds__(kThreads)
         const float d2 = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2];
         const float inv = 1.0f / sqrtf(d2);   // glm::normalize: v * inversesqrt(dot(v, v))
         SphSurfaceVertex o;
-        o.position[0] = p[0];
-        o.position[1] = p[1];
-        o.position[2] = p[2];
+        o.position[0] = pnt[0];
+        o.position[1] = pnt[1];
+        o.position[2] = pnt[2];
         o.normal = pack_unit(n[0] * inv, n[1] * inv, n[2] * inv);
-        out[slot + v] = o;
+        out[base + k] = o;
     }
 }
 
 int upload_table() {
     const int8_t* t = sphhost::mc_table();
     if (!t) return fail(SPH_ERR_STATE, "marching-cubes table generation failed");
+    uint8_t nverts[256];
+    for (uint32_t c = 0; c < 256; ++c) {
+        uint32_t v = 0;
+        while (v < 16u && t[16u * c + v] >= 0) ++v;
+        nverts[c] = (uint8_t)v;
+    }
     SPH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_table), t, 256 * 16));
+    SPH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_nverts), nverts, 256));
     return SPH_OK;
+}
+
+// Workgroups of the marching passes, and the device words march_on_device works in: two
+// counters, the workgroups' vertex offsets (one more for the scan) and the scan's tile sums.
+uint32_t march_blocks(uint32_t resolution) {
+    const size_t cells = (size_t)resolution * resolution * resolution;
+    return (uint32_t)((cells + kThreads - 1) / kThreads);
+}
+
+size_t march_scratch_words(uint32_t resolution) {
+    const uint32_t blocks = march_blocks(resolution);
+    const uint32_t tiles = (blocks + sphdev::kScanTile - 1) / sphdev::kScanTile;
+    return 2 + ((size_t)blocks + 1) + ((size_t)tiles + 1);
 }
 
 // Field of the solver's current state into the device buffer `field` ((res + 1)^3 floats).
@@ -385,12 +464,16 @@ int field_on_device(sphhost::Engine& e, const SphParams& p, uint32_t resolution,
     return SPH_OK;
 }
 
-// `counters`: two device words, overwritten.
+// `scratch`: march_scratch_words(resolution) device words, overwritten.
 int march_on_device(const float* field, uint32_t resolution, const float origin[3], const float cellSize[3],
-                    float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written, uint32_t* counters,
+                    float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written, uint32_t* scratch,
                     hipStream_t s) {
     if (int st = upload_table()) return st;
     const uint32_t init[2] = {0u, capacity};
+    uint32_t* counters = scratch;
+    uint32_t* blockStart = scratch + 2;
+    const uint32_t blocks = march_blocks(resolution);
+    uint32_t* tileSums = blockStart + blocks + 1;
     hipError_t he = hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, s);
     MarchArgs m;
     for (int k = 0; k < 3; ++k) {
@@ -400,9 +483,13 @@ int march_on_device(const float* field, uint32_t resolution, const float origin[
     m.iso = iso;
     m.resolution = resolution;
     m.capacity = capacity;
-    const size_t cells = (size_t)resolution * resolution * resolution;
     if (he == hipSuccess) {
-        k_march<<<(uint32_t)((cells + kThreads - 1) / kThreads), kThreads, 0, s>>>(field, m, out, counters);
+        k_march_count<<<blocks, kThreads, 0, s>>>(field, m, blockStart);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = sphdev::launch_scan(blockStart, blocks, 0u, tileSums, s);
+    if (he == hipSuccess) {
+        k_march_emit<<<blocks, kThreads, 0, s>>>(field, m, blockStart, blocks, out, counters);
         he = hipGetLastError();
     }
     uint32_t got[2] = {0u, 0u};
@@ -413,8 +500,8 @@ int march_on_device(const float* field, uint32_t resolution, const float origin[
     return SPH_OK;
 }
 
-// The solver's field and vertex buffers, grown to at least these sizes (never shrunk).
-int ensure_surface_buffers(SphSolver* solver, size_t fieldBytes, uint32_t vertices) {
+// The solver's field, vertex and marching buffers, grown to at least these sizes (never shrunk).
+int ensure_surface_buffers(SphSolver* solver, size_t fieldBytes, uint32_t vertices, size_t marchWords) {
     if (solver->surfFieldBytes < fieldBytes) {
         hipFree(solver->surfField);
         solver->surfField = nullptr;
@@ -429,7 +516,13 @@ int ensure_surface_buffers(SphSolver* solver, size_t fieldBytes, uint32_t vertic
         SPH_HIP(hipMalloc(&solver->surfVerts, (size_t)vertices * sizeof(SphSurfaceVertex)));
         solver->surfVertCap = vertices;
     }
-    if (!solver->surfCounters) SPH_HIP(hipMalloc(&solver->surfCounters, 2 * sizeof(uint32_t)));
+    if (solver->surfMarchWords < marchWords) {
+        hipFree(solver->surfMarch);
+        solver->surfMarch = nullptr;
+        solver->surfMarchWords = 0;
+        SPH_HIP(hipMalloc(&solver->surfMarch, marchWords * sizeof(uint32_t)));
+        solver->surfMarchWords = marchWords;
+    }
     return SPH_OK;
 }
 
@@ -452,7 +545,7 @@ extern "C" int sph_surface_field(SphSolver* solver, const SphParams* p, uint32_t
     if (int st = e.set_device()) return st;
     const uint32_t s = resolution + 1;
     const size_t bytes = (size_t)s * s * s * sizeof(float);
-    if (int st = ensure_surface_buffers(solver, bytes, 0)) return st;
+    if (int st = ensure_surface_buffers(solver, bytes, 0, 0)) return st;
     int st = field_on_device(e, *p, resolution, solver->surfField);
     if (!st) {
         hipError_t he = hipMemcpyAsync(field, solver->surfField, bytes, hipMemcpyDeviceToHost, e.stream);
@@ -482,7 +575,7 @@ extern "C" int sph_march_field(const float* field, uint32_t resolution, const fl
     uint32_t* counters = nullptr;
     SPH_HIP(hipMalloc(&d, bytes));
     hipError_t he = hipMalloc(&v, (size_t)capacity * sizeof(SphSurfaceVertex));
-    if (he == hipSuccess) he = hipMalloc(&counters, 2 * sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMalloc(&counters, march_scratch_words(resolution) * sizeof(uint32_t));
     int st = SPH_OK;
     if (he == hipSuccess) he = hipMemcpy(d, field, bytes, hipMemcpyHostToDevice);
     if (he != hipSuccess) st = hip_fail(he, "sph_march_field");
@@ -512,7 +605,9 @@ extern "C" int sph_surface_mesh(SphSolver* solver, const SphParams* p, uint32_t 
     if (int st = e.set_params(0, *p)) return st;
     if (int st = e.set_device()) return st;
     const uint32_t s = resolution + 1;
-    if (int st = ensure_surface_buffers(solver, (size_t)s * s * s * sizeof(float), capacity)) return st;
+    if (int st = ensure_surface_buffers(solver, (size_t)s * s * s * sizeof(float), capacity,
+                                        march_scratch_words(resolution)))
+        return st;
     float* d = solver->surfField;
     SphSurfaceVertex* v = solver->surfVerts;
     int st = field_on_device(e, *p, resolution, d);
@@ -521,7 +616,7 @@ extern "C" int sph_surface_mesh(SphSolver* solver, const SphParams* p, uint32_t 
         origin[k] = p->containerMin[k];
         cell[k] = (p->containerMax[k] - p->containerMin[k]) / (float)resolution;
     }
-    if (!st) st = march_on_device(d, resolution, origin, cell, iso, v, capacity, written, solver->surfCounters, e.stream);
+    if (!st) st = march_on_device(d, resolution, origin, cell, iso, v, capacity, written, solver->surfMarch, e.stream);
     if (!st && *written) {
         const hipError_t he = hipMemcpy(out, v, (size_t)*written * sizeof(SphSurfaceVertex), hipMemcpyDeviceToHost);
         if (he != hipSuccess) st = hip_fail(he, "sph_surface_mesh");

@@ -1,7 +1,7 @@
 """GPU parity of the fluid surface: the field bit for bit against the oracle's
-buildFluidField restatement, and marching cubes as the same set of triangles (vertex
-positions and packed normals bit for bit; the order of triangles in the buffer is not
-deterministic on the GPU, as in the reference's threaded loop)."""
+buildFluidField restatement, and marching cubes as the oracle's serial marchFluidField
+restatement writes it: the same vertices (positions and packed normals bit for bit) in the
+same order, and with too small a buffer the same whole-cell prefix."""
 import ctypes as C
 import sys
 
@@ -118,7 +118,14 @@ def test_march_matches_the_oracle(sph, oracle_lib):
     assert len(want_pos) > 300
     got_pos, got_nrm = _march_gpu(field, res, origin, cell, iso, 1 << 20)
     assert len(got_pos) == len(want_pos)
-    assert _triangles(got_pos, got_nrm) == _triangles(want_pos, want_nrm)
+    assert np.array_equal(got_pos.view(np.uint32), want_pos.view(np.uint32))
+    assert np.array_equal(got_nrm, want_nrm)
+    # buffers that end inside a cell's vertices: the serial loop's prefix of whole cells
+    for cap in (3, 4, 101, len(want_pos) - 1):
+        gp, gn = _march_gpu(field, res, origin, cell, iso, cap)
+        wp, wn = oracle_lib.march_field(field, res, origin, cell, iso, _table(), capacity=cap)
+        assert len(gp) == len(wp), cap
+        assert np.array_equal(gp.view(np.uint32), wp.view(np.uint32)) and np.array_equal(gn, wn), cap
 
 
 def test_mesh_in_one_call_and_capacity(sph, oracle_lib):
@@ -128,16 +135,20 @@ def test_mesh_in_one_call_and_capacity(sph, oracle_lib):
     field = oracle_lib.surface_field(pos, p, res)
     origin = list(p.containerMin)
     cell = [(F32(p.containerMax[a]) - F32(p.containerMin[a])) / F32(res) for a in range(3)]
-    want = _triangles(*oracle_lib.march_field(field, res, origin, cell, iso, _table()))
+    want_pos, want_nrm = oracle_lib.march_field(field, res, origin, cell, iso, _table())
+    want = _triangles(want_pos, want_nrm)
     cap = 1 << 20
     buf = (SphSurfaceVertex * cap)()
     written = C.c_uint32()
     capi.check(capi.load().sph_surface_mesh(s._h, C.byref(p), res, iso, buf, cap, C.byref(written)))
     say(f"mesh: {written.value} vertices")
-    assert _triangles(*_vertices(buf, written.value)) == want
-    # too small a buffer: whole triangles only, all of them from the full set
+    got_pos, got_nrm = _vertices(buf, written.value)
+    assert np.array_equal(got_pos.view(np.uint32), want_pos.view(np.uint32)) and np.array_equal(got_nrm, want_nrm)
+    # too small a buffer: whole cells only, the serial loop's prefix
     small = 3 * (len(want) // 3) + 2
     buf2 = (SphSurfaceVertex * small)()
     capi.check(capi.load().sph_surface_mesh(s._h, C.byref(p), res, iso, buf2, small, C.byref(written)))
     assert written.value <= small and written.value % 3 == 0 and written.value > 0
-    assert set(_triangles(*_vertices(buf2, written.value))) <= set(want)
+    wp, wn = oracle_lib.march_field(field, res, origin, cell, iso, _table(), capacity=small)
+    gp, gn = _vertices(buf2, written.value)
+    assert np.array_equal(gp.view(np.uint32), wp.view(np.uint32)) and np.array_equal(gn, wn)
