@@ -114,6 +114,9 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // ones the visit would give). Summing inside the visit would run the ~45 instructions of
 // the sums for every candidate that any lane of the wave has in range; at 1M particles in a
 // developed flow this pass took 0.54 ms that way against 0.31 ms for the pass without sums.
+// With `deviation` it also writes each workgroup's sums of max(rho / rho0 - 1, 0) and
+// |rho / rho0 - 1| (the density check's first stage), in the tree the check's own reduction
+// uses, so the check needs no pass of its own.
 // DEDUPE = false is the variant for launches where no scene asked for de-duplicated
 // buckets: it keeps no bucket hashes live, which leaves the reference-mode pass at 64 VGPRs
 // (8 waves per SIMD, 4 VGPRs spilled, measured faster than one wave fewer) and the pass
