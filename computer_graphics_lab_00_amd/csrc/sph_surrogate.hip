@@ -643,18 +643,20 @@ namespace {
 // The check's buffers: mapped host memory for its two means and one workgroup sum pair per
 // workgroup of the solver's launches.
 int ensure_check(SphNet* n, sphhost::Engine& e) {
-    const sphdev::Launch L = e.plan_launch();
+    // A single-scene solver's launches have one workgroup per kBlock particles; the plan may
+    // not exist yet (a fresh solver), so the count comes from the particles, not the plan.
+    const uint32_t blocks = (e.sc[0].n + sphdev::kBlock - 1) / sphdev::kBlock;
     if (!n->hostCheck) {
         SPH_HIP(hipHostMalloc(&n->hostCheck, 2 * sizeof(double), hipHostMallocMapped));
         SPH_HIP(hipHostGetDevicePointer((void**)&n->hostCheckDev, n->hostCheck, 0));
         SPH_HIP(hipDeviceSynchronize());
     }
-    if (n->devPartialCap < L.blocks) {
+    if (n->devPartialCap < blocks) {
         hipFree(n->devPartial);
         n->devPartial = nullptr;
         n->devPartialCap = 0;
-        SPH_HIP(hipMalloc(&n->devPartial, (size_t)L.blocks * sizeof(float2)));
-        n->devPartialCap = L.blocks;
+        SPH_HIP(hipMalloc(&n->devPartial, (size_t)blocks * sizeof(float2)));
+        n->devPartialCap = blocks;
     }
     return SPH_OK;
 }
@@ -664,6 +666,7 @@ int ensure_check(SphNet* n, sphhost::Engine& e) {
 int launch_check(SphNet* n, sphhost::Engine& e, bool summed) {
     const sphdev::Launch L = e.plan_launch();
     if (int rc = ensure_check(n, e)) return rc;
+    if (L.blocks > n->devPartialCap) return fail(SPH_ERR_STATE, "density check: more workgroups than partial sums");
     n->hostCheck[0] = 0.0;
     n->hostCheck[1] = 0.0;
     const uint32_t count = e.sc[0].n;

@@ -259,6 +259,24 @@ def test_density_check_equals_the_oracle(sph, oracle_lib):
     assert want[1] > 0
 
 
+def test_rollout_check_equals_the_separate_check(sph):
+    """The check a rollout reads (workgroup sums written by its density pass with sums, then
+    added in FP64) equals sph_density_deviation's own reduction of the same state."""
+    w = fluid.m1()   # 8 substeps per step; the lattice starts ~1 % compressed: a non-zero check
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    s = fluid.FluidSolver(w.count, p.cellCount)
+    s.load_state(pos, vel)
+    dt_sub = float(F32(F32(1.0 / 600.0) / F32(fluid.fluid_substeps(p, 1.0 / 600.0))))
+    net = _trained_net()
+    say("rollout check == separate check")
+    st = surrogate.rollout(s, net, p, 1.0 / 600.0, steps=1, threshold=float("inf"))
+    assert st.macroSteps == 2 and st.fallbacks == 0
+    got = surrogate.density_deviation(s, net, p, dt_sub)
+    say(f"rollout {st.lastDeviation!r} separate {got[0]!r}")
+    assert got[0] > 0 and F32(st.lastDeviation) == F32(got[0])
+
+
 def test_rollout_with_every_check_failing_is_the_plain_path(sph):
     p, pos, vel, s = _small_scene(seed=5)
     say("rollout threshold<0 == plain")
