@@ -122,11 +122,54 @@ def bench_batch64(args) -> None:
     print(json.dumps(line), flush=True)
 
 
+def host_cpus() -> dict:
+    """The host's CPUs as the reference's JobSystem sees them (hardware_concurrency(): the
+    online CPUs, src/core/job_system.cpp:16-18) and the share this process may use: its
+    affinity mask, the cgroup CPU quota and the thread budget the environment sets
+    (OMP_NUM_THREADS). The baseline runs hardware_concurrency() threads, as the reference
+    does, unless the usable share is smaller: more threads than CPUs only time-slice."""
+    online = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = online
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    budget = None
+    try:
+        budget = int(os.environ["OMP_NUM_THREADS"])
+    except (KeyError, ValueError):
+        pass
+    share = affinity
+    if quota:
+        share = min(share, max(1, int(quota)))
+    if budget:
+        share = min(share, budget)
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"hardware_concurrency": online, "affinity": affinity, "cgroup_quota": quota,
+            "omp_num_threads": budget, "threads": max(1, min(online, share)), "model": model}
+
+
 def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
     """The oracle's FP32 restatement of FluidSolver::step on this host (test
-    infrastructure, used here only as the reported baseline)."""
+    infrastructure, used here only as the reported baseline), with the reference's
+    JobSystem thread count (hardware_concurrency(), capped by the CPU share this process
+    is given) and 256-particle chunks."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    cpus = host_cpus()
+    threads = cpus["threads"]
     s = O.OracleSolver(32)
     s.load_state(pos, vel)
     substeps = 8
@@ -148,6 +191,7 @@ def cpu_baseline(w, p, pos, vel, budget_s: float) -> dict:
         "sample": f"{done} substeps (= {done / substeps:.3g} step(1/600) calls) of the same 1M-particle "
                   f"dam-break state, FP32 reference-semantics restatement, {threads} threads, {el:.1f} s",
         "particle_substeps_per_s": rate_sub,
+        "host": cpus,
     }
 
 

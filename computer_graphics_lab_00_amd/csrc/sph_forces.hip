@@ -151,7 +151,9 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
     for (int q = 0; q < 20; ++q) ax[q] = 0.0f;
     float vx = 0.0f, vy = 0.0f, vz = 0.0f, invH2 = 0.0f, invH = 0.0f;
     if (SUMS) {
-        const float4 v = vel[t];
+        // Lanes past the scene's particles stay alive only for the workgroup reduction:
+        // they read nothing (the state arrays end at the last slot).
+        const float4 v = live ? vel[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         vx = v.x;
         vy = v.y;
         vz = v.z;

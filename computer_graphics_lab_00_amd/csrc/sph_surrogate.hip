@@ -83,8 +83,10 @@ struct SphNet {
     bool timed[2] = {false, false};
     // Engine epochs (unique in the process): the solver's sorted buffers and `sums` hold
     // the prepared state of epoch `preparedEpoch`, and `iy` the network's prediction for
-    // the prepared state of epoch `predictedEpoch` (0: none).
+    // the prepared state of epoch `predictedEpoch` (0: none). The launch tables of a prepared
+    // state carry its substep length (dt and the speed clamp of the integrate): `preparedDt`.
     uint64_t preparedEpoch = 0, predictedEpoch = 0;
+    float preparedDt = 0.0f;
     // The inference buffers (sums, ix, iy, keep*, the check's) are used on the stream of the
     // solver a call works with; a call on another solver's stream first waits for `done`,
     // recorded at the end of the previous call on `lastStream`.
@@ -804,9 +806,12 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
                 st.backedOff += 1;
                 continue;
             }
-            if (net->preparedEpoch != e.epoch()) {
+            // A prepared state of another substep length (the previous call's frame time
+            // differed) has the wrong dt and speed clamp in its launch tables: prepare again.
+            if (net->preparedEpoch != e.epoch() || net->preparedDt != dtSub) {
                 if (int rc = e.prepare(dtSub, &ns)) return rc;
                 net->preparedEpoch = e.epoch();
+                net->preparedDt = dtSub;
             }
             if (net->predictedEpoch != e.epoch())
                 if (int rc = predict(net, e, 1u - net->predictedSlot)) return rc;
@@ -819,6 +824,7 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             e.commit_sorted();
             if (int rc = e.prepare(dtSub, &ns)) return rc;
             net->preparedEpoch = e.epoch();
+            net->preparedDt = dtSub;
             if (int rc = launch_check(net, e, true)) return rc;
             // the next macro-step's prediction, queued before the host waits for the check
             if (m + 1 < macros || rest == 0)

@@ -362,8 +362,10 @@ int sph_mc_table(int8_t* out);
 /* buildFluidField for the solver's current particles into the host array `field`. */
 int sph_surface_field(SphSolver* solver, const SphParams* params, uint32_t resolution, float* field);
 /* marchFluidField over a host field: at most `capacity` vertices (three per triangle) into
- * `out`, their count in *written (the reference's capacity rule). Vertex order is not
- * deterministic; the set of triangles is. */
+ * `out`, their count in *written (the reference's capacity rule). Vertices come out in
+ * the reference's serial loop order (cells x fastest, each cell's triangles in table
+ * order), so the buffer equals the reference's array entry for entry; a cell whose
+ * vertices would pass `capacity` writes none and ends the list, as the serial loop does. */
 int sph_march_field(const float* field, uint32_t resolution, const float origin[3], const float cellSize[3],
                     float iso, SphSurfaceVertex* out, uint32_t capacity, uint32_t* written);
 /* Both on the device for the solver's current particles (grid over the container). */

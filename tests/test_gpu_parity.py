@@ -6,6 +6,7 @@ order with IEEE rounding, no contraction). The 1e-5 relative tolerance on densit
 force is asserted separately so a failure says which bar broke. Each
 result's error against the FP64 oracle is printed.
 """
+import math
 import os
 import sys
 
@@ -290,6 +291,57 @@ def test_colliders_bit_exact(oracle_lib):
         assert d.min() > 0.3 - 1e-2
 
 
+def _rotated(shape, radius, half_y, center, axis, angle):
+    """A collider of `shape` rotated by `angle` about axis 'x' or 'z', then moved to `center`."""
+    c, s_ = math.cos(angle), math.sin(angle)
+    if axis == "z":
+        R = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]], np.float32)
+    else:
+        R = np.array([[1, 0, 0], [0, c, -s_], [0, s_, c]], np.float32)
+    t = np.array(center, np.float32)
+    W = np.eye(4, dtype=np.float32)
+    W[:3, :3] = R
+    W[:3, 3] = t
+    Wi = np.eye(4, dtype=np.float32)
+    Wi[:3, :3] = R.T
+    Wi[:3, 3] = -R.T @ t
+    col = SphCollider()
+    col.shape = shape
+    col.radius = radius
+    col.halfExtents[:] = [radius, half_y, radius]
+    col.world[:] = list(W.T.reshape(-1))      # column-major
+    col.inverseWorld[:] = list(Wi.T.reshape(-1))
+    return col
+
+
+@pytest.mark.parametrize("shape", ["cylinder", "capsule"])
+def test_cylinder_and_capsule_colliders_bit_exact(oracle_lib, shape):
+    """The cylinder and capsule branches of closestOnColliderLocal
+    (src/physics/collider_shapes.h:75-112), tilted so the fluid meets caps, rims and sides:
+    40 frames bit-exact against the oracle, and the collider visibly deflects the fluid."""
+    say(f"case: {shape} collider")
+    if shape == "cylinder":
+        col = _rotated(capi.SHAPE_CYLINDER, 0.22, 0.08, (0.03, 0.55, -0.02), "z", 0.5)
+    else:
+        col = _rotated(capi.SHAPE_CAPSULE, 0.12, 0.18, (-0.04, 0.5, 0.03), "x", 1.1)
+    p = fluid.derive_fluid_params(SphSettings(), fluid.translation4(0.0, 1.0, 0.0), 2048, 4096, colliders=[col])
+    free = fluid.derive_fluid_params(SphSettings(), fluid.translation4(0.0, 1.0, 0.0), 2048, 4096)
+    assert p.colliderCount == 1
+    with fluid.FluidSolver(2048, 4096) as gpu, fluid.FluidSolver(2048, 4096) as without:
+        gpu.emit(p, 2048)
+        without.emit(free, 2048)
+        ora = oracle_lib.OracleSolver(32)
+        ora.emit(p, 2048)
+        for _ in range(40):
+            gpu.step(p, 1.0 / 60.0)
+            without.step(free, 1.0 / 60.0)
+            ora.step(p, 1.0 / 60.0, THREADS)
+        assert_bit_equal(gpu.particles(), ora.particles(), shape)
+        moved = np.abs(gpu.particles()[:, :3] - without.particles()[:, :3]).max(axis=1)
+        say(f"  particles deflected by more than 1 cm: {(moved > 0.01).sum()}")
+        assert (moved > 0.01).sum() > 100
+
+
 def test_force_fields_within_tolerance(oracle_lib):
     """Wind, vortex and point fields. powf of the falloff is libm on the host and the
     device library on the GPU, so this case is held to 1e-5, not to bits."""
@@ -353,6 +405,32 @@ def test_m1_clamped_cellcount_one_substep(oracle_lib):
     counts = ora.grid(65536)[1]
     assert counts.max() > 32
     compare_substep(gpu, ora, p, "m1 C=65536")
+    gpu.close()
+
+
+@pytest.mark.parametrize("cells", [1 << 21, 65536])
+def test_m1_developed_state_one_substep(cells, oracle_lib):
+    """One substep bit-exact from the 1M dam break after 100 steps (800 substeps) of the
+    HIP path itself: a disordered, compressed state where buckets run past 32 members
+    and the collapse has piled particles against the floor and walls."""
+    say(f"case: m1 developed state (100 steps), cellCount {cells}")
+    w = fluid.m1(cells)
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    with fluid.FluidSolver(w.count, cells) as run:
+        run.load_state(pos, vel)
+        for _ in range(100):
+            run.step(p, 1.0 / 600.0)
+        pos, vel = run.state()
+    dt = 1.0 / 600.0 / 8
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    counts = ora.grid(cells)[1]
+    say(f"  buckets over 32 members: {(counts > 32).sum()}, largest {counts.max()}, "
+        f"density range {pos[:, 3].min():.1f}..{pos[:, 3].max():.1f}")
+    if cells == 65536:
+        assert counts.max() > 32
+    compare_substep(gpu, ora, p, f"m1 developed C={cells}")
+    fp64_report(p, pos, vel, dt, gpu, oracle_lib, f"m1 developed C={cells}")
     gpu.close()
 
 

@@ -487,24 +487,46 @@ def _macro_gravity_only(p, pos, vel, dt_sub, max_speed, accel=None):
     return x, v
 
 
-def test_rollout_equals_features_forward_and_restated_macro_steps(sph):
+def _xsph(p, x, v, smoothing, c_poly6):
+    """The rollout's XSPH step restated: v += s * (sum s w / sum s) from the density pass's
+    neighbourhood sums (the oracle's walk: column 11 = sum s, 12-14 = sum s w), where the
+    sum of weights is positive."""
+    _, sums = SO.walk(x, v, p.smoothingRadius, p.cellCount, p.particleMass, c_poly6)
+    wsum = sums[:, 11]
+    pos_w = wsum > 0
+    safe = np.where(pos_w, wsum, F32(1))
+    adj = F32(smoothing) * (sums[:, 12:15] / safe[:, None])
+    out = v.copy()
+    out[:, :3] = np.where(pos_w[:, None], v[:, :3] + adj, v[:, :3])
+    return out
+
+
+@pytest.mark.parametrize("lattice,smoothing", [((16, 16, 8), 0.0), ((16, 16, 8), surrogate.DEFAULT_SMOOTHING),
+                                               ((20, 10, 10), surrogate.DEFAULT_SMOOTHING)])
+def test_rollout_equals_features_forward_and_restated_macro_steps(sph, lattice, smoothing):
     """The rollout builds its feature rows inside the fused network kernel; restated
-    with the separate kernels instead (sph_features -> sph_net_forward) and the macro-step
-    in numpy, two macro-steps of the trained network match bit for bit."""
-    p, pos, vel, s = _small_scene(seed=10)
-    say("rollout == features + forward + restated macro-steps")
+    with the separate kernels instead (sph_features -> sph_net_forward), the XSPH step from
+    the oracle's neighbourhood sums and the macro-step in numpy, two macro-steps of the
+    trained network match bit for bit — at the shipped smoothing too, and at 2,000
+    particles (not a multiple of the 256-lane workgroup: the last workgroup's idle lanes)."""
+    p, pos, vel, s = _small_scene(seed=10, lattice=lattice)
+    n = pos.shape[0]
+    say(f"rollout == features + forward + restated macro-steps, n={n}, smoothing={smoothing}")
     net = surrogate.SurrogateNet(seed=3)
     net.load(surrogate.default_weights())
+    net.set_smoothing(smoothing)
     dt_sub = F32(F32(1.0 / 600.0) / F32(8))
     consts = (C.c_float * 8)()
     capi.check(capi.load().sph_debug_step_constants(C.byref(p), float(dt_sub), consts))
     scale = F32(98.1)
     x, v = pos.copy(), vel.copy()
-    probe = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    probe = fluid.FluidSolver(n, 1 << 14)
     for _ in range(2):
         probe.load_state(x, v)
         y = net.forward(surrogate.features(probe, p, float(dt_sub)))[:, :3]
         assert np.abs(y).max() > 0.01
+        if smoothing > 0:
+            v = _xsph(p, x, v, smoothing, consts[2])
         xn, vn = _macro_gravity_only(p, x, v, dt_sub, F32(consts[5]), accel=y.astype(F32) * scale)
         x = np.c_[xn, np.zeros(len(xn), F32)]
         v = np.c_[vn, np.zeros(len(vn), F32)]
@@ -513,6 +535,28 @@ def test_rollout_equals_features_forward_and_restated_macro_steps(sph):
     gp, gv = s.state()
     np.testing.assert_array_equal(gp[:, :3], x[:, :3])
     np.testing.assert_array_equal(gv[:, :3], v[:, :3])
+
+
+def test_rollout_after_a_frame_time_change_uses_the_new_substep(sph):
+    """A rollout call re-uses the state its previous call prepared; when the frame time
+    changes between calls (1/600 s, then 1/500 s) the macro-steps must run with the new
+    substep length and speed clamp: bit for bit what a fresh solver loaded with the same
+    state does at 1/500 s."""
+    p, pos, vel, a = _small_scene(seed=13)
+    say("rollout 1/600 then 1/500 == fresh solver at 1/500")
+    inf = float("inf")
+    net = _trained_net()
+    net.set_smoothing(surrogate.DEFAULT_SMOOTHING)
+    surrogate.rollout(a, net, p, 1.0 / 600.0, steps=1, threshold=inf)
+    mid_pos, mid_vel = a.state()
+    st = surrogate.rollout(a, net, p, 1.0 / 500.0, steps=1, threshold=inf)
+    assert st.macroSteps == 2 and st.fallbacks == 0
+    b = fluid.FluidSolver(pos.shape[0], 1 << 14)
+    b.load_state(mid_pos, mid_vel)
+    other = _trained_net()
+    other.set_smoothing(surrogate.DEFAULT_SMOOTHING)
+    surrogate.rollout(b, other, p, 1.0 / 500.0, steps=1, threshold=inf)
+    assert _same_state(a, b)
 
 
 def test_zero_network_rollout_is_the_gravity_only_macro_step(sph):

@@ -350,6 +350,65 @@ def test_density_bucket_cap_hides_particles_past_32(oracle_lib):
     assert np.allclose(s.state()[0][:, 3], 27 * p.particleMass * w.sum(1), rtol=1e-12)
 
 
+def _single_bucket_forces_fp64(pos, vel, p, visible):
+    """Closed form of the force pass when every one of the 27 neighbour buckets is the one
+    bucket (cellCount = 1) holding the particles `visible` (the first 32 by index): each
+    visible pair is summed 27 times (src/physics/fluid_sph.cpp:234-258), so
+    F_i = 27 sum_{j visible, j != i, 1e-6 < r < h} [-m (p_i + p_j) / (2 rho_j) grad W_spiky
+    + mu m (v_j - v_i) / rho_j lap W_visc], with rho_i = 27 m sum_{j visible} W_poly6."""
+    x = pos[:, :3].astype(np.float64)
+    v = vel[:, :3].astype(np.float64)
+    hh = np.float64(np.float32(p.smoothingRadius))
+    m, k, mu, rho0 = (np.float64(np.float32(a)) for a in (p.particleMass, p.stiffness, p.viscosity, p.restDensity))
+    delta = x[:, None, :] - x[None, :, :]
+    r2 = (delta ** 2).sum(-1)
+    r = np.sqrt(r2)
+    vis = np.zeros(len(x), bool)
+    vis[visible] = True
+    W = np.where((r2 < hh * hh) & vis[None, :], 315.0 / (64 * np.pi * hh ** 9) * (hh * hh - r2) ** 3, 0.0)
+    rho = np.maximum(27 * m * W.sum(1), 1e-6)
+    pr = np.maximum(k * (rho - rho0), 0.0)
+    pair = vis[None, :] & (r < hh) & (r > 1e-6) & ~np.eye(len(x), dtype=bool)
+    safe_r = np.where(pair, r, 1.0)
+    grad = np.where(pair, -45.0 / (np.pi * hh ** 6) * (hh - r) ** 2 / safe_r, 0.0)[:, :, None] * delta
+    sp = -m * (pr[:, None] + pr[None, :]) / (2 * rho[None, :])
+    lap = np.where(pair, 45.0 / (np.pi * hh ** 6) * (hh - r), 0.0)
+    visc = mu * m * (v[None, :, :] - v[:, None, :]) / rho[None, :, None] * lap[:, :, None]
+    return 27 * ((sp[:, :, None] * grad).sum(1) + visc.sum(1)), rho
+
+
+@pytest.mark.parametrize("n", [20, 40])
+def test_force_single_bucket_closed_form(oracle_lib, n):
+    """cellCount = 1 pins the force pass's duplicate visits (27 x every visible pair) and,
+    with 40 particles, its 32-slot cap: particles past index 31 push no one but are pushed
+    by the first 32."""
+    pos = _cloud(n, 8 + n, 0.04)
+    rng = np.random.default_rng(n)
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(0, 0.3, (n, 3))
+    p = params_for_points(h=0.1, cell_count=1, mass=0.02, k=50.0, mu=0.5)
+    want, rho = _single_bucket_forces_fp64(pos, vel, p, np.arange(min(n, 32)))
+    assert (rho > 1000.0).any()   # the pressure term is live
+    s = oracle_lib.OracleSolver(64)
+    s.load_state(pos, vel)
+    s.substep(p, 1e-6)
+    got = s.forces()[:, :3]
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 1e-10 * scale
+    # the FP32 restatement within float rounding of 27 x 31 sequential additions
+    s32 = oracle_lib.OracleSolver(32)
+    s32.load_state(pos, vel)
+    s32.substep(p, 1e-6)
+    assert np.abs(s32.forces()[:, :3] - want).max() <= 1e-4 * scale
+    if n > 32:
+        # a particle past the cap is visible to no one: moving it changes no other force
+        moved = pos.copy()
+        moved[n - 1, :3] += np.float32(0.01)
+        s.load_state(moved, vel)
+        s.substep(p, 1e-6)
+        assert np.array_equal(s.forces()[: n - 1, :3], got[: n - 1])
+
+
 def test_tait_eos_properties(oracle_lib):
     """The build-only Tait option: p(rho0) = 0, monotone, slope k at rest density."""
     h = 0.1
