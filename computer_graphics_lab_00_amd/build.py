@@ -34,6 +34,7 @@ SOURCES = [
     "sph_surrogate.hip",
     "sph_surface.hip",
     "sph_output.hip",
+    "sph_noalloc.hip",
 ]
 
 COMMON_FLAGS = [

@@ -263,6 +263,8 @@ def _declare(lib):
         "sph_net_set_smoothing": (C.c_int, [P, C.c_float]),
         "sph_net_set_backoff": (C.c_int, [P, C.c_uint32]),
         "sph_density_deviation": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, _DP]),
+        "sph_substep_device": (C.c_int, [C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

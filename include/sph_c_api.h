@@ -463,6 +463,26 @@ int sph_net_inference_time(SphNet* net, double* ms, uint64_t* rows);
  * state, from a density pass of a substep of length dtSub. */
 int sph_density_deviation(SphSolver* solver, SphNet* net, const SphParams* params, float dtSub, double out[2]);
 
+/* ---- allocation-free substep ----
+ * No counterpart in the reference. For callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. One
+ * FluidSolver::substep (src/physics/fluid_sph.cpp:190-309) of length dt — the results of
+ * sph_substep on a solver holding the same state, mode and equation of state, bit for
+ * bit — on caller-owned DEVICE buffers: posIn / velIn hold `count` AoS vec4 in particle
+ * order (the particles() layout), posOut / velOut receive the next state in the same
+ * layout (they may be posIn / velIn). Enqueued on `stream` (a hipStream_t; NULL is the
+ * default stream) and returns before the device finishes; allocates nothing and copies
+ * nothing from host memory. The buckets, bins, sort scratch and sorted state live in one
+ * static device array of SPH_NOALLOC_SCRATCH_BYTES per device (held until the process
+ * ends), so count is at most SPH_NOALLOC_MAX_PARTICLES and params->cellCount at most
+ * SPH_NOALLOC_MAX_CELLS (larger values return SPH_ERR_CAPACITY). Every call on a device
+ * shares that array: calls that overlap on one device race and must be serialised. */
+#define SPH_NOALLOC_MAX_PARTICLES 65536u
+#define SPH_NOALLOC_MAX_CELLS 65536u
+#define SPH_NOALLOC_SCRATCH_BYTES 8392448u
+int sph_substep_device(const SphParams* params, float dt, uint32_t modeFlags, uint32_t eos, const float* posIn,
+                       const float* velIn, float* posOut, float* velOut, uint32_t count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
