@@ -36,6 +36,7 @@ EOS_LINEAR, EOS_TAIT = 0, 1
 CFG_KEEP_FORCES = 1
 CFG_NO_GRAPH = 2
 CFG_BUCKET_WALK = 4
+CFG_TILED_DENSITY = 8
 
 
 class SphCollider(C.Structure):

@@ -29,6 +29,7 @@ constexpr uint32_t kScanTile = kBlock * kScanPerThread;
 constexpr uint32_t kXcds = 8;             // MI355X: 8 XCDs, each with its own L2
 constexpr uint32_t kPosPad = 8;           // slots of padding after each position array: the
                                           // density pass reads candidates eight at a time
+constexpr uint32_t kHalfTileBins = 32;    // spatial bins per workgroup of the tiled density pass
 
 // Scalars of one substep of one scene, precomputed on the host exactly as the reference
 // computes them (src/physics/fluid_sph.cpp:24-46, :194-195, :264, :271-272) so device and
@@ -84,7 +85,7 @@ struct SceneDesc {
     uint32_t binCapacity;   // bins reserved; the filler bin binOffset + binCapacity holds capacity - n
     uint32_t blockOffset;   // first workgroup of the scene in particle-parallel launches
     uint32_t active;        // 0: this substep is past the scene's substep count; state passes through
-    uint32_t pad;
+    uint32_t tileOffset;    // first workgroup of the scene in half-tile launches (binCapacity / 32 of them)
 };
 
 // Colliders and force fields of one scene (at most 8 each).
@@ -104,6 +105,8 @@ struct Launch {
     uint32_t totalCells;
     uint32_t totalBins;           // sum of (binCapacity + 1)
     uint32_t modes;               // OR of the scenes' SPH_MODE_* flags (picks kernel variants)
+    const uint16_t* tileScene = nullptr;   // scene of each half-tile workgroup (32 spatial bins)
+    uint32_t tiles = 0;                    // half-tile workgroups over all scenes (0: none planned)
 };
 
 struct GridBuffers {
@@ -124,6 +127,10 @@ struct GridBuffers {
     uint32_t* idsSorted;
     uint32_t* keysSorted;
     uint32_t* bucketStart;  // totalCells
+    uint32_t* restCount = nullptr;   // when set: zeroed with the counts (the tiled density pass's list)
+    // For the tiled density pass (set when the launch has half-tiles): the list of particles
+    // whose own cell lies outside the half-tile they are sorted into.
+    uint2* rest = nullptr;           // (slot, scene)
 };
 
 // Logical block for a hardware workgroup id. Workgroups are dispatched round-robin over
@@ -185,10 +192,21 @@ hipError_t launch_emit(const SphParams& p, uint32_t count, float4* pos, float4* 
 hipError_t launch_iota(uint32_t* ids, uint32_t n, hipStream_t s);
 hipError_t launch_calibration_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 
+// The tiled density pass (sph_forces.hip): a workgroup per half-tile of 32 spatial bins
+// keeps the bucket tables of the cells around them in LDS; the particles the grid build
+// listed in `rest` (bucket members whose own cell lies outside their half-tile) are walked in
+// global memory by `restBlocks` further workgroups of the same launch.
+struct DensityTiles {
+    const uint32_t* binStart;   // the scanned bin starts of the grid build
+    const uint2* rest;          // (slot, scene) per listed particle
+    const uint32_t* restCount;
+    uint32_t restBlocks;
+};
+
 // launchers (sph_forces.hip)
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
                           const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
-                          const NeighbourSums* sums, hipStream_t s);
+                          const NeighbourSums* sums, hipStream_t s, const DensityTiles* tiles = nullptr);
 hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
                         const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s);

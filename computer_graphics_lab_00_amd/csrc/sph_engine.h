@@ -134,6 +134,10 @@ private:
     sphdev::SceneDesc* dTables = nullptr;
     sphdev::BoundaryConsts* dBounds = nullptr;
     uint16_t* dBlockScene = nullptr;
+    uint16_t* dTileScene = nullptr;   // scene of each half-tile workgroup of the tiled density pass
+    uint32_t maxTiles = 0, planTiles = 0;
+    uint2* restList = nullptr;        // particles the tiled density pass leaves to its global walk
+    uint32_t* restCount = nullptr;
     void* hostStage = nullptr;
     size_t hostStageBytes = 0;
     std::vector<uint8_t> planKey;

@@ -54,6 +54,10 @@ float libm_powf(float x, float y) {
 
 uint32_t blocks_for(uint32_t n) { return (n + sphdev::kBlock - 1) / sphdev::kBlock; }
 
+// Workgroups of the tiled density launch that walk the particles the grid build listed
+// (a multiple of the 8 XCDs, so the half-tiles after them keep their XCD mapping).
+constexpr uint32_t kRestBlocks = 256;
+
 }  // namespace
 
 int hip_fail(hipError_t e, const char* what) {
@@ -203,6 +207,9 @@ void Engine::free_all() {
     hipFree(dTables);
     hipFree(dBounds);
     hipFree(dBlockScene);
+    hipFree(dTileScene);
+    hipFree(restList);
+    hipFree(restCount);
     if (hostStage) hipHostFree(hostStage);
     hostStage = nullptr;
     for (auto& t : timed) {
@@ -231,7 +238,7 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     if (caps.empty() || caps.size() > 65535) return fail(SPH_ERR_INVALID_ARGUMENT, "scene count must be 1..65535");
     device = dev;
     flags = fl;
-    uint64_t tn = 0, tc = 0, tb = 0, blk = 0;
+    uint64_t tn = 0, tc = 0, tb = 0, blk = 0, halfTiles = 0;
     sc.resize(caps.size());
     for (size_t i = 0; i < caps.size(); ++i) {
         const uint32_t n = caps[i].first, c = caps[i].second;
@@ -250,6 +257,7 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
         tc += c;
         tb += nb + 1;
         blk += blocks_for(n);
+        halfTiles += nb / sphdev::kHalfTileBins;
     }
     if (tn > 0x7FFFFFFFull || tc > 0x7FFFFFFFull || tb > 0x7FFFFFFFull)
         return fail(SPH_ERR_CAPACITY, "total capacity above 2^31 slots");
@@ -257,6 +265,7 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     totalC = (uint32_t)tc;
     totalBins = (uint32_t)tb;
     maxBlocks = (uint32_t)blk;
+    maxTiles = (uint32_t)halfTiles;
     const size_t N = totalN, C = totalC, B = totalBins, S = sc.size();
     const size_t tiles = (B + sphdev::kScanTile - 1) / sphdev::kScanTile;
     hipError_t e = hipSetDevice(device);
@@ -290,8 +299,13 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     if (e == hipSuccess) e = hipMalloc(&dTables, kMaxSubsteps * S * sizeof(sphdev::SceneDesc));
     if (e == hipSuccess) e = hipMalloc(&dBounds, S * sizeof(sphdev::BoundaryConsts));
     if (e == hipSuccess) e = hipMalloc(&dBlockScene, (maxBlocks + 1) * sizeof(uint16_t));
+    if (e == hipSuccess && (flags & SPH_CFG_TILED_DENSITY)) {
+        e = hipMalloc(&dTileScene, ((size_t)maxTiles + 1) * sizeof(uint16_t));
+        if (e == hipSuccess) e = hipMalloc(&restList, N * sizeof(uint2));
+        if (e == hipSuccess) e = hipMalloc(&restCount, sizeof(uint32_t));
+    }
     hostStageBytes = kMaxSubsteps * S * sizeof(sphdev::SceneDesc) + S * sizeof(sphdev::BoundaryConsts) +
-                     (maxBlocks + 1) * sizeof(uint16_t) + 256;
+                     (maxBlocks + 1) * sizeof(uint16_t) + ((size_t)maxTiles + 1) * sizeof(uint16_t) + 512;
     if (e == hipSuccess) e = hipHostMalloc(&hostStage, hostStageBytes, hipHostMallocDefault);
     if (e != hipSuccess) {
         int st = e == hipErrorOutOfMemory ? SPH_ERR_OUT_OF_MEMORY : SPH_ERR_DEVICE;
@@ -375,6 +389,8 @@ sphdev::Launch Engine::launch_for(uint32_t j) const {
     L.totalBins = planBins;
     L.modes = 0u;
     for (const SceneHost& s : sc) L.modes |= s.mode;
+    L.tileScene = dTileScene;
+    L.tiles = dTileScene ? planTiles : 0u;
     return L;
 }
 
@@ -430,7 +446,8 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     sphdev::SceneDesc* tables = reinterpret_cast<sphdev::SceneDesc*>(h);
     sphdev::BoundaryConsts* bounds = reinterpret_cast<sphdev::BoundaryConsts*>(h + kMaxSubsteps * S * sizeof(sphdev::SceneDesc));
     uint16_t* blockScene = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(bounds) + S * sizeof(sphdev::BoundaryConsts));
-    uint32_t blocks = 0;
+    uint16_t* tileScene = blockScene + (maxBlocks + 1);
+    uint32_t blocks = 0, tiles = 0;
     uint32_t binBase = 0;   // bins are packed by the bins each scene actually uses
     for (size_t i = 0; i < S; ++i) {
         const SceneHost& s = sc[i];
@@ -438,6 +455,8 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
         const sphdev::BinGrid grid = s.n ? make_bins(s, s.params) : sphdev::BinGrid{};
         const uint32_t sceneBins = s.n ? grid.bins : 0u;
         for (uint32_t q = 0; q < b; ++q) blockScene[blocks + q] = (uint16_t)i;
+        const uint32_t sceneTiles = sceneBins / sphdev::kHalfTileBins;   // bins come in tiles of 64
+        for (uint32_t q = 0; q < sceneTiles; ++q) tileScene[tiles + q] = (uint16_t)i;
         sphdev::BoundaryConsts& bc = bounds[i];
         memset(&bc, 0, sizeof(bc));
         if (s.n) {
@@ -453,6 +472,7 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
             d.binOffset = binBase;
             d.binCapacity = sceneBins;
             d.blockOffset = blocks;
+            d.tileOffset = tiles;
             d.active = j < sub[i] ? 1u : 0u;
             if (s.n) {
                 d.k = make_consts(s, s.params, dts[i], nbrList != nullptr);
@@ -467,15 +487,33 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
             d.k.n = s.n;
         }
         blocks += b;
+        tiles += sceneTiles;
         binBase += sceneBins + 1u;
     }
     planBlocks = blocks;
+    // The tiled density pass needs one bin per cell and a neighbourhood table of at most 256
+    // cells in every scene; scenes with far fewer buckets than particles would send most
+    // particles to its global walk. Otherwise the whole launch uses the plain walk.
+    bool tileable = dTileScene != nullptr;
+    for (size_t i = 0; i < S && tileable; ++i) {
+        const SceneHost& sh = sc[i];
+        if (sh.n == 0) continue;
+        const sphdev::BinGrid& gg = tables[i].g;
+        uint32_t dm[3] = {1u << gg.tileLog2[0], 1u << gg.tileLog2[1], 1u << gg.tileLog2[2]};
+        dm[gg.tileLog2[2] ? 2 : (gg.tileLog2[1] ? 1 : 0)] >>= 1;
+        const uint32_t box = (dm[0] + 2u) * (dm[1] + 2u) * (dm[2] + 2u);
+        if (gg.shift != 0u || box > 256u || sh.params.cellCount < sh.n) tileable = false;
+    }
+    planTiles = tileable ? tiles : 0u;
     planBins = binBase;
     planSub = maxSub;
     SPH_HIP(hipMemcpyAsync(dTables, tables, (size_t)maxSub * S * sizeof(sphdev::SceneDesc), hipMemcpyHostToDevice, stream));
     SPH_HIP(hipMemcpyAsync(dBounds, bounds, S * sizeof(sphdev::BoundaryConsts), hipMemcpyHostToDevice, stream));
     SPH_HIP(hipMemcpyAsync(dBlockScene, blockScene, (size_t)(blocks + 1) * sizeof(uint16_t), hipMemcpyHostToDevice,
                            stream));
+    if (planTiles)
+        SPH_HIP(hipMemcpyAsync(dTileScene, tileScene, (size_t)planTiles * sizeof(uint16_t), hipMemcpyHostToDevice,
+                               stream));
     SPH_HIP(hipStreamSynchronize(stream));
     planKey.swap(key);
     return SPH_OK;
@@ -501,6 +539,8 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
     g.idsSorted = ids[1 - ic];
     g.keysSorted = keysSorted;
     g.bucketStart = bucketStart;
+    g.restCount = restCount;
+    g.rest = restList;
     {
         Timer t(this, KC_FILL);
         SPH_HIP(sphdev::launch_grid_clear(L, g, stream));
@@ -530,8 +570,10 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
         Timer t(this, KC_DENSITY);
         // A prepare-only pass is followed by no force pass here: no neighbour list.
         const bool list = !prepareOnly;
+        sphdev::DensityTiles tiles{binStart, restList, restCount, kRestBlocks};
         SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, list ? nbrList : nullptr,
-                                       list ? nbrCount : nullptr, sums, stream));
+                                       list ? nbrCount : nullptr, sums, stream,
+                                       (restList && !sums) ? &tiles : nullptr));
     }
     if (prepareOnly) return SPH_OK;
     {

@@ -280,6 +280,289 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
     }
 }
 
+// ---- tiled density pass ----
+//
+// The plain step's density pass (fluid_sph.cpp:197-222) for launches whose scenes all use
+// one spatial bin per cell. One workgroup per half-tile: 32 spatial bins, i.e. a block of
+// 32 cells (4 x 4 x 2 for the default Morton tile) and the particles sorted into them. The
+// workgroup first loads, for every cell of the block and of the cells around it
+// (6 x 6 x 4 = 144 cells), its bucket's first slot, member count (capped at 32 unless
+// uncapped) and the bounding box of the members the walk visits into LDS: the 27 lookups
+// per particle of the plain walk become about one per particle, and lanes of one cell read
+// the same LDS entries.
+//
+// Culling: a bucket whose bounding box is at least h away from the particle cannot hold a
+// candidate in range, and is skipped. This is exact, not a tolerance: with g the gap between
+// the particle and the box per axis, computed as px - hi (or lo - px), and
+// G2 = (gx gx + gy gy) + gz gz in the visit's order, every member q has
+// |fl(px - qx)| >= gx per axis (rounding is monotone) and therefore a computed r2 >= G2;
+// G2 >= h2 means every member's r2 >= h2, and a candidate out of range adds nothing to the
+// sum (poly6 is 0 there and density + 0 == density). The density and the neighbour list
+// are therefore the ones the full walk gives, bit for bit. At rest about 40 % of the
+// candidates survive.
+//
+// Each lane then walks its kept buckets in the reference's order at its own pace, four
+// candidates per iteration read from the sorted state in global memory, so a lane does not
+// wait at every bucket for the wave-mate with the largest one.
+//
+// Particles whose own cell lies outside the block they are sorted into (members of a
+// bucket shared with a cell elsewhere: a bucket's members sit in the bin of its smallest
+// cell) were listed by the grid build; the launch's first `restBlocks` workgroups walk them
+// as the plain pass does, while the half-tiles run.
+constexpr uint32_t kBoxCells = 256;    // cells of the neighbourhood table, at most
+constexpr uint32_t kRing = 16;         // pending neighbour-list entries per lane (batches of 8)
+
+// The in-range slots of one batch of B candidates (bit e of m: slot base + e) join the list
+// in order through the lane's ring of kRing rows (row kRing takes the writes of the slots that
+// do not join); completed groups of four are written out. At most 3 entries are pending before
+// a batch, so kRing >= B + 3 rows hold them. Past `cap` entries are only counted (the force
+// pass then walks the buckets itself).
+template <uint32_t B>
+__device__ __forceinline__ void list_push(uint32_t (*ring)[kBlock], uint32_t m, uint32_t base, uint32_t cap,
+                                          uint32_t& listed, uint32_t& flushed, uint4* __restrict__ nbrList,
+                                          size_t stride, uint32_t t) {
+    static_assert(B + 3 <= kRing, "ring too small for the batch");
+#pragma unroll
+    for (uint32_t e = 0; e < B; ++e) {
+        const uint32_t at = listed + (uint32_t)__builtin_popcount(m & ((1u << e) - 1u));
+        const bool put = ((m >> e) & 1u) != 0u && at < cap;
+        ring[put ? (at & (kRing - 1u)) : kRing][threadIdx.x] = base + e;
+    }
+    listed += (uint32_t)__builtin_popcount(m);
+    const uint32_t done = listed < cap ? listed : cap;
+#pragma unroll
+    for (uint32_t g = 0; g < (B + 3) / 4; ++g) {
+        if (flushed + 4u <= done) {
+            const uint32_t r0 = flushed & (kRing - 1u);
+            store_stream(nbrList + (size_t)(flushed >> 2) * stride + t,
+                         make_uint4(ring[r0][threadIdx.x], ring[r0 + 1u][threadIdx.x], ring[r0 + 2u][threadIdx.x],
+                                    ring[r0 + 3u][threadIdx.x]));
+            flushed += 4u;
+        }
+    }
+}
+
+// The density, pressure and list count of particle t once its walk is done.
+__device__ __forceinline__ void density_finish(const StepConsts& k, float density, float4* __restrict__ pos,
+                                               float4* __restrict__ vel, uint32_t t, uint32_t cap, uint32_t listed,
+                                               uint32_t flushed, uint32_t (*ring)[kBlock],
+                                               uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
+                                               size_t stride) {
+    density = density < 1e-6f ? 1e-6f : density;
+    reinterpret_cast<float*>(pos + t)[3] = density;
+    reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
+    if (nbrCount) {
+        if (listed <= cap && flushed < listed) {
+            const uint32_t r0 = flushed & (kRing - 1u), r = listed - flushed;
+            store_stream(nbrList + (size_t)(flushed >> 2) * stride + t,
+                         make_uint4(ring[r0][threadIdx.x], r > 1u ? ring[r0 + 1u][threadIdx.x] : 0u,
+                                    r > 2u ? ring[r0 + 2u][threadIdx.x] : 0u, 0u));
+        }
+        nbrCount[t] = cap && listed <= cap ? listed : cap + 1u;
+    }
+}
+
+// Candidates base[0..B-1] (valid: index < validCount) of particle (px, py, pz): the
+// reference's density term for each in range, in order; returns the in-range mask. The reads
+// may run up to B - 1 slots past the bucket (the position arrays are padded by kPosPad).
+template <uint32_t B>
+__device__ __forceinline__ uint32_t density_batch(const StepConsts& k, float px, float py, float pz,
+                                                  const float4* __restrict__ base, uint32_t validCount,
+                                                  float& density) {
+    static_assert(B <= kPosPad, "reads past a bucket stay inside the padding");
+    float qx[B], qy[B], qz[B];
+#pragma unroll
+    for (uint32_t e = 0; e < B; ++e) {
+        const float* q = reinterpret_cast<const float*>(base + e);
+        qx[e] = q[0];
+        qy[e] = q[1];
+        qz[e] = q[2];
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < B; ++e) {
+        const float ddx = px - qx[e], ddy = py - qy[e], ddz = pz - qz[e];
+        const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        const bool inr = e < validCount && r2 < k.h2;
+        const float d = k.h2 - r2;
+        const float term = k.mass * (((k.cPoly6 * d) * d) * d);
+        density = inr ? density + term : density;
+        m |= inr ? (1u << e) : 0u;
+    }
+    return m;
+}
+
+// The plain walk of one listed particle (slot t of scene d), in global memory.
+__device__ __forceinline__ void density_walk_global(const SceneDesc* d, uint32_t t, float4* __restrict__ pos,
+                                                    float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
+                                                    const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList,
+                                                    uint32_t* __restrict__ nbrCount, uint32_t (*ring)[kBlock],
+                                                    size_t stride) {
+    const StepConsts k = d->k;
+    const uint32_t cellBase = d->cellOffset;
+    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+    const uint32_t cap = nbrList ? k.listCap : 0u;
+    const float* self = reinterpret_cast<const float*>(pos + t);
+    const float px = self[0], py = self[1], pz = self[2];
+    const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
+    float density = 0.0f;
+    uint32_t listed = 0, flushed = 0;
+    for (int32_t dz = -1; dz <= 1; ++dz) {
+        uint32_t cn[9], st[9];
+#pragma unroll
+        for (uint32_t o = 0; o < 9; ++o) {
+            const uint32_t b = cellBase + hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz,
+                                                    k.cellMask);
+            cn[o] = counts[b];
+            st[o] = bucketStart[b];
+        }
+        for (uint32_t o = 0; o < 9; ++o) {
+            uint32_t cnt = cn[o];
+            if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+            for (uint32_t j = 0; j < cnt; j += 8) {
+                const uint32_t m = density_batch<8>(k, px, py, pz, pos + st[o] + j, cnt - j, density);
+                list_push<8>(ring, m, st[o] + j, cap, listed, flushed, nbrList, stride, t);
+            }
+        }
+    }
+    density_finish(k, density, pos, vel, t, cap, listed, flushed, ring, nbrList, nbrCount, stride);
+}
+
+__global__ void __launch_bounds__(kBlock, 8)
+    k_density_tiled(Launch L, const uint32_t* __restrict__ binStart, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
+                    const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
+                    const uint2* __restrict__ rest, const uint32_t* __restrict__ restCount) {
+    __shared__ float4 boxLo[kBoxCells];   // per neighbourhood cell: bucket box low corner, .w = first slot
+    __shared__ float4 boxHi[kBoxCells];   // bucket box high corner, .w = member count (capped)
+    __shared__ uint32_t ring[kRing + 1][kBlock];
+
+    // The first restBlocks workgroups (dispatched first, so they run beside the half-tiles)
+    // walk the listed particles, one per lane; the others are the half-tiles.
+    const uint32_t restBlocks = gridDim.x - L.tiles;
+    if (blockIdx.x < restBlocks) {
+        const uint32_t n = *restCount;
+        for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += restBlocks * kBlock) {
+            const uint2 r = rest[i];
+            const SceneDesc* d = L.scenes + r.y;
+            if (d->active)
+                density_walk_global(d, r.x, pos, vel, bucketStart, counts, nbrList, nbrCount, ring, L.total);
+        }
+        return;
+    }
+    const uint32_t lb = xcd_block(blockIdx.x - restBlocks, L.tiles);
+    const uint32_t sIdx = L.tileScene[lb];
+    const SceneDesc* d = L.scenes + sIdx;
+    if (!d->active) return;
+    const uint32_t ht = lb - d->tileOffset;
+    const uint32_t b0 = d->binOffset + ht * kHalfTileBins;
+    const uint32_t s0 = binStart[b0], s1 = binStart[b0 + kHalfTileBins];
+    if (s0 >= s1) return;   // an empty half-tile (the same for every lane)
+    const StepConsts k = d->k;
+    const uint32_t cellBase = d->cellOffset;
+    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+    const uint32_t cap = nbrList ? k.listCap : 0u;
+
+    // The half-tile's cells: tile (tx, ty, tz) of 2^tileLog2 bins per axis, halved along the
+    // axis that holds bit 5 of the in-tile bin number (z for the Morton tile).
+    const uint32_t lx = d->g.tileLog2[0], ly = d->g.tileLog2[1], lz = d->g.tileLog2[2];
+    const uint32_t tile = ht >> 1, half = ht & 1u;
+    int32_t dim[3] = {1 << lx, 1 << ly, 1 << lz};
+    int32_t org[3] = {(int32_t)((tile % d->g.tiles[0]) << lx), (int32_t)(((tile / d->g.tiles[0]) % d->g.tiles[1]) << ly),
+                      (int32_t)((tile / (d->g.tiles[0] * d->g.tiles[1])) << lz)};
+    const int ha = lz ? 2 : (ly ? 1 : 0);
+    dim[ha] >>= 1;
+    org[ha] += (int32_t)half * dim[ha];
+    // the neighbourhood: the block's cells and one more on each side
+    const int32_t bx0 = d->g.origin[0] + org[0] - 1, by0 = d->g.origin[1] + org[1] - 1, bz0 = d->g.origin[2] + org[2] - 1;
+    const uint32_t BX = (uint32_t)dim[0] + 2u, BY = (uint32_t)dim[1] + 2u, BZ = (uint32_t)dim[2] + 2u;
+    const uint32_t BXY = BX * BY, nbox = BXY * BZ;   // <= kBoxCells: the planner checks the tile shape
+
+    // One thread per neighbourhood cell: its bucket's first slot, member count and the
+    // bounding box of the members the walk visits (read from the sorted state, which also
+    // brings them into the caches the lanes read them from next).
+    const uint32_t kc = threadIdx.x;
+    if (kc < nbox) {
+        const int32_t cx = bx0 + (int32_t)(kc % BX), cy = by0 + (int32_t)((kc / BX) % BY), cz = bz0 + (int32_t)(kc / BXY);
+        const uint32_t b = cellBase + hash_cell(cx, cy, cz, k.cellMask);
+        uint32_t cnt = counts[b];
+        if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t first = 0;
+        if (cnt) {
+            first = bucketStart[b];
+            for (uint32_t m = 0; m < cnt; m += 4) {
+                float q[4][3];
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    const float* c = reinterpret_cast<const float*>(pos + first + m + e);   // padded
+                    q[e][0] = c[0];
+                    q[e][1] = c[1];
+                    q[e][2] = c[2];
+                }
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    if (m + e < cnt) {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            lo[a] = fminf(lo[a], q[e][a]);
+                            hi[a] = fmaxf(hi[a], q[e][a]);
+                        }
+                    }
+                }
+            }
+        }
+        boxLo[kc] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(first));
+        boxHi[kc] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(cnt));
+    }
+    __syncthreads();
+
+    for (uint32_t t = s0 + threadIdx.x; t < s1; t += kBlock) {
+        const float* self = reinterpret_cast<const float*>(pos + t);
+        const float px = self[0], py = self[1], pz = self[2];
+        const int32_t rx = cell_coord(px, k.h) - bx0 - 1, ry = cell_coord(py, k.h) - by0 - 1,
+                      rz = cell_coord(pz, k.h) - bz0 - 1;
+        // a particle of another cell's bucket: listed by the grid build, walked elsewhere
+        if (!(rx >= 0 && rx < dim[0] && ry >= 0 && ry < dim[1] && rz >= 0 && rz < dim[2])) continue;
+        // the 27 neighbour cells in the reference's order: the one at offset (dx, dy, dz),
+        // each in 0..2, is table entry kb0 + dz BXY + dy BX + dx
+        const uint32_t kb0 = (uint32_t)rz * BXY + (uint32_t)ry * BX + (uint32_t)rx;
+        uint32_t keep = 0;
+#pragma unroll
+        for (uint32_t o = 0; o < 27; ++o) {
+            const uint32_t kb = kb0 + (o / 9u) * BXY + ((o / 3u) % 3u) * BX + (o % 3u);
+            const float4 lo = boxLo[kb], hi = boxHi[kb];
+            const float gx = fmaxf(fmaxf(lo.x - px, px - hi.x), 0.0f);
+            const float gy = fmaxf(fmaxf(lo.y - py, py - hi.y), 0.0f);
+            const float gz = fmaxf(fmaxf(lo.z - pz, pz - hi.z), 0.0f);
+            const float g2 = (gx * gx + gy * gy) + gz * gz;
+            // an empty cell has an infinite box: g2 = inf
+            keep |= (uint32_t)(g2 < k.h2) << o;
+        }
+        float density = 0.0f;
+        uint32_t listed = 0, flushed = 0;
+        // cursor: the kept bucket being walked (first slot, member count) and the position in it
+        uint32_t cnt = 0, start = 0, j = 0;
+        auto open = [&](uint32_t o) {
+            const uint32_t dz = (o * 57u) >> 9, r = o - 9u * dz, dy = (r * 11u) >> 5, dx = r - 3u * dy;
+            const uint32_t kb = kb0 + dz * BXY + dy * BX + dx;
+            start = __float_as_uint(boxLo[kb].w);
+            cnt = __float_as_uint(boxHi[kb].w);
+            j = 0;
+        };
+        if (keep) open((uint32_t)__builtin_ctz(keep));
+        while (keep) {
+            const uint32_t m = density_batch<8>(k, px, py, pz, pos + start + j, cnt - j, density);
+            list_push<8>(ring, m, start + j, cap, listed, flushed, nbrList, L.total, t);
+            j += 8u;
+            if (j >= cnt) {
+                keep &= keep - 1u;
+                if (keep) open((uint32_t)__builtin_ctz(keep));
+            }
+        }
+        density_finish(k, density, pos, vel, t, cap, listed, flushed, ring, nbrList, nbrCount, L.total);
+    }
+}
+
 // ---- boundary pieces (src/physics/force_field.h:31-70, collider_shapes.h:33-131) ----
 
 struct Surface {
@@ -602,7 +885,7 @@ __global__ void __launch_bounds__(kBlock, 8)
 
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
                           const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
-                          const NeighbourSums* sums, hipStream_t s) {
+                          const NeighbourSums* sums, hipStream_t s, const DensityTiles* tiles) {
     if (L.blocks == 0) return hipSuccess;
     const bool dedupe = (L.modes & SPH_MODE_DEDUPE) != 0;
     if (sums) {
@@ -612,6 +895,10 @@ hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint3
         else
             k_density<true, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nullptr, nullptr,
                                                                sums->sums, sums->deviation);
+    } else if (tiles && !dedupe && L.tiles > 0 && L.tileScene) {
+        k_density_tiled<<<L.tiles + tiles->restBlocks, kBlock, 0, s>>>(L, tiles->binStart, pos, vel,
+                                                                       bucketStart, counts, nbrList, nbrCount,
+                                                                       tiles->rest, tiles->restCount);
     } else {
         if (dedupe)
             k_density<false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,

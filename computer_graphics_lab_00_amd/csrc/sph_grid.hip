@@ -77,8 +77,9 @@ __device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ul
 // counts = 0 and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
 __global__ void __launch_bounds__(kBlock)
     k_grid_clear(uint32_t* __restrict__ counts, uint32_t* __restrict__ rep, uint32_t cells,
-                 uint32_t* __restrict__ binCount, uint32_t bins) {
+                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount) {
     const uint32_t stride = gridDim.x * kBlock;
+    if (restCount && blockIdx.x == 0 && threadIdx.x == 0) *restCount = 0u;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride) {
         counts[i] = 0u;
         rep[i] = 0xFFFFFFFFu;
@@ -254,18 +255,25 @@ __global__ void __launch_bounds__(kBlock)
 // Final slot = bin start + number of bin members before this one in (bucket, id) order.
 // Thread `slot` handles the pair scattered to that slot; a scene's n particles fill
 // exactly its first n slots.
+// With `rest` (a launch with half-tiles for the tiled density pass), every particle whose own
+// cell lies outside the half-tile (32 bins) it is sorted into — a member of a bucket shared
+// with another cell — is listed there.
 __global__ void __launch_bounds__(kBlock)
     k_rank_reorder(Launch L, const uint2* __restrict__ pairKI, const uint32_t* __restrict__ pairSrc,
                    const uint32_t* __restrict__ binOf, const uint32_t* __restrict__ binStart,
                    const float4* __restrict__ pos, const float4* __restrict__ vel, float4* __restrict__ posSorted,
                    float4* __restrict__ velSorted, uint32_t* __restrict__ idsSorted,
-                   uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart) {
+                   uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart, uint2* __restrict__ rest,
+                   uint32_t* __restrict__ restCount) {
     const Where w = locate(L);
-    if (!w.valid) return;
-    const uint2 me = pairKI[w.slot];
-    const uint32_t src = pairSrc[w.slot];
-    const uint32_t bin = binOf[src];
-    const uint32_t lo = binStart[bin], hi = binStart[bin + 1];
+    const uint2 me = w.valid ? pairKI[w.slot] : make_uint2(0u, 0u);
+    const uint32_t src = w.valid ? pairSrc[w.slot] : 0u;
+    const uint32_t bin = w.valid ? binOf[src] : 0u;
+    uint32_t lo = 0, hi = 0;
+    if (w.valid) {
+        lo = binStart[bin];
+        hi = binStart[bin + 1];
+    }
     uint32_t rank = 0;
     bool first = true;
     for (uint32_t u = lo; u < hi; ++u) {
@@ -275,11 +283,41 @@ __global__ void __launch_bounds__(kBlock)
         if (sameKey && o.y < me.y) first = false;
     }
     const uint32_t dst = lo + rank;
-    posSorted[dst] = pos[src];
-    velSorted[dst] = vel[src];
-    idsSorted[dst] = me.y;
-    keysSorted[dst] = me.x;
-    if (first) bucketStart[me.x] = dst;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (w.valid) {
+        p = pos[src];
+        posSorted[dst] = p;
+        velSorted[dst] = vel[src];
+        idsSorted[dst] = me.y;
+        keysSorted[dst] = me.x;
+        if (first) bucketStart[me.x] = dst;
+    }
+    if (!rest) return;   // the same for every lane
+    // Is the particle's own cell inside the half-tile its bin belongs to? (bins are cells when
+    // shift == 0; the tiled pass is planned only then)
+    bool out = false;
+    if (w.valid) {
+        const BinGrid& g = w.d->g;
+        const float h = w.d->k.h;
+        const int32_t c[3] = {cell_coord(p.x, h), cell_coord(p.y, h), cell_coord(p.z, h)};
+        bool inGrid = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int64_t q = (int64_t)c[a] - g.origin[a];
+            inGrid = inGrid && q >= 0 && q < (int64_t)g.dims[a];
+        }
+        out = !inGrid ||
+              (bin_of(g, c[0], c[1], c[2]) / kHalfTileBins) != ((bin - w.d->binOffset) / kHalfTileBins);
+    }
+    const uint64_t outs = __ballot(out);
+    if (outs) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)outs) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(restCount, (uint32_t)__popcll(outs));
+        base = __shfl(base, (int)leader, 64);
+        if (out) rest[base + (uint32_t)__popcll(outs & ((1ull << lane) - 1ull))] = make_uint2(dst, w.d - L.scenes);
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -347,7 +385,7 @@ hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t 
     const uint32_t span = L.totalCells > L.totalBins ? L.totalCells : L.totalBins;
     uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
-    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins);
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount);
     return hipGetLastError();
 }
 
@@ -378,8 +416,10 @@ hipError_t launch_scatter(const Launch& L, const GridBuffers& b, hipStream_t s) 
 }
 
 hipError_t launch_rank_reorder(const Launch& L, const GridBuffers& b, hipStream_t s) {
+    const bool tiles = L.tiles > 0 && b.rest && b.restCount;
     k_rank_reorder<<<L.blocks, kBlock, 0, s>>>(L, b.pairKI, b.pairSrc, b.binOf, b.binStart, b.pos, b.vel,
-                                              b.posSorted, b.velSorted, b.idsSorted, b.keysSorted, b.bucketStart);
+                                              b.posSorted, b.velSorted, b.idsSorted, b.keysSorted, b.bucketStart,
+                                              tiles ? b.rest : nullptr, b.restCount);
     return hipGetLastError();
 }
 

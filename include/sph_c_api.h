@@ -156,6 +156,9 @@ typedef struct SphSolverConfig {
 #define SPH_CFG_NO_GRAPH 2u     /* launch substeps eagerly instead of replaying a captured hipGraph */
 #define SPH_CFG_BUCKET_WALK 4u  /* force pass walks the 27 buckets instead of the density pass's
                                  * neighbour list (saves 256 B of device memory per particle) */
+#define SPH_CFG_TILED_DENSITY 8u  /* density pass by blocks of 32 cells: the block's neighbourhood
+                                   * bucket tables in LDS, buckets culled per particle by their
+                                   * bounding boxes (same results; measured slower, see DESIGN) */
 
 /* ---- stateless helpers ---- */
 

@@ -42,12 +42,36 @@ def run_alone(spec, steps):
         return s.state()
 
 
-def test_batch_equals_scenes_run_alone():
+@pytest.mark.parametrize("flags", [0, capi.CFG_TILED_DENSITY])
+@pytest.mark.parametrize("tile", [(2, 2, 2), (4, 2, 0), (6, 0, 0), (0, 0, 6), (1, 2, 3)])
+def test_tile_shapes_match_the_oracle(tile, flags, oracle_lib):
+    """The spatial-bin tile shape only moves buckets in memory; the tiled density pass
+    keeps each half-tile's neighbourhood table by that shape (a shape whose neighbourhood
+    is too large makes the launch use the plain walk). One step(1/600) of a scene with
+    each shape equals the oracle bit for bit, with either density pass."""
+    spec = batch.batch64(5, scenes=1, n_min=12000, n_max=12001)[0]
+    spec.tile = tile
+    say(f"case: tile {tile}, {spec.count} particles, flags {flags}")
+    p = spec.params()
+    pos, vel = spec.initial_state(p)
+    with batch.SceneBatch([spec], flags=flags) as b:
+        b.load_state(0, pos, vel)
+        b.step(DT)
+        gp, gv = b.state(0)
+    ora = oracle_lib.OracleSolver(32)
+    ora.load_state(pos, vel)
+    ora.step(p, DT, 8)
+    op, ov = ora.state()
+    assert np.array_equal(bits(gp), bits(op)) and np.array_equal(bits(gv), bits(ov))
+
+
+@pytest.mark.parametrize("flags", [0, capi.CFG_TILED_DENSITY])
+def test_batch_equals_scenes_run_alone(flags):
     specs = small_specs()
     subs = [fluid.fluid_substeps(s.params(), DT) for s in specs]
     say(f"case: batch of {len(specs)} scenes, substeps {subs}")
     assert len(set(subs)) > 1, "the batch should mix substep counts"
-    with batch.SceneBatch(specs) as b:
+    with batch.SceneBatch(specs, flags=flags) as b:
         b.load_initial()
         for _ in range(4):
             b.step(DT)

@@ -138,7 +138,8 @@ def test_golden_fixture(name):
 
 
 @pytest.mark.parametrize("cells,flags", [(32768, 0), (1024, 0), (32768, capi.CFG_BUCKET_WALK),
-                                         (1024, capi.CFG_BUCKET_WALK)])
+                                         (1024, capi.CFG_BUCKET_WALK), (32768, capi.CFG_TILED_DENSITY),
+                                         (1024, capi.CFG_TILED_DENSITY)])
 def test_s16k_one_substep(cells, flags, oracle_lib):
     say(f"case: s16k one substep, cellCount {cells}, flags {flags}")
     w = fluid.s16k(cells)
@@ -380,7 +381,7 @@ def test_gpu_runs_are_bitwise_deterministic():
     assert_bit_equal(outs[0][1], outs[1][1], "run-to-run velocities")
 
 
-@pytest.mark.parametrize("flags", [0, capi.CFG_BUCKET_WALK])
+@pytest.mark.parametrize("flags", [0, capi.CFG_BUCKET_WALK, capi.CFG_TILED_DENSITY])
 def test_m1_one_substep(flags, oracle_lib):
     """1,048,576 particles, 2^21 buckets: grid and one substep bit-exact; FP64 error."""
     say(f"case: m1 one substep, cellCount 2^21, flags {flags}")
