@@ -56,6 +56,7 @@ def parse():
                     help="dam_break_1m: the headline line; batch64: 64 independent scenes in one hipGraph")
     ap.add_argument("--tune", action="store_true", help="batch64: re-run the autotuner instead of its cache")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--no-fast", action="store_true", help="skip the fast-math leg")
     ap.add_argument("--surrogate-steps", type=int, default=30)
     return ap.parse_args()
 
@@ -242,6 +243,40 @@ def surrogate_leg(w, steps: int) -> dict | None:
     return out
 
 
+def profile_window(solver, p, pos_w, vel_w, steps: int) -> dict:
+    """Per-kernel event times of `steps` steps replayed eagerly from a saved state."""
+    solver.load_state(pos_w, vel_w)
+    solver.profile_enable(True)
+    solver.profile_read()   # reset
+    for _ in range(steps):
+        solver.step(p, DT)
+    solver.synchronize()
+    prof = solver.profile_read()
+    solver.profile_enable(False)
+    return prof
+
+
+def fast_math_leg(n, p, pos_w, vel_w, steps: int, flags: int) -> dict:
+    from computer_graphics_lab_00_amd import capi, fluid
+    with fluid.FluidSolver(n, max(p.cellCount, 1024), flags=flags) as s:
+        s.set_mode(capi.MODE_FAST_MATH)
+        s.load_state(pos_w, vel_w)
+        for _ in range(2):   # plan and graph capture
+            s.step(p, DT)
+        s.load_state(pos_w, vel_w)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step(p, DT)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        prof = profile_window(s, p, pos_w, vel_w, steps)
+    return {"value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el * 1e3 / steps,
+            "mode": "SPH_MODE_FAST_MATH: same grid and neighbour semantics, density/force within 1e-5 "
+                    "of the reference after one substep (not bit-exact)",
+            "kernels_ms_per_substep": {k: v[0] / max(v[1], 1) for k, v in prof.items()}}
+
+
 def load_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (counters scaled by a plain copy kernel of known size), or None."""
@@ -288,12 +323,15 @@ def main():
     print(f"[bench] workload {w.name}: {n} particles, {p.cellCount} buckets, {substeps} substeps per step",
           file=sys.stderr, flush=True)
 
-    gpu = fluid.FluidSolver(n, max(p.cellCount, 1024), flags=capi.CFG_BUCKET_WALK if args.bucket_walk else 0)
+    flags = capi.CFG_BUCKET_WALK if args.bucket_walk else 0
+    gpu = fluid.FluidSolver(n, max(p.cellCount, 1024), flags=flags)
     gpu.load_state(pos, vel)
     for _ in range(args.warmup):
         gpu.step(p, DT)
     gpu.synchronize()
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    # the state the timed window starts from, kept to replay the same window for the profile
+    pos_w, vel_w = gpu.state()
 
     # The timed region replays the captured step graph, as a caller would run it.
     gpu.synchronize()
@@ -303,17 +341,11 @@ def main():
     gpu.synchronize()
     elapsed = time.perf_counter() - t0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-    # Per-kernel durations: the next steps of the same run, launched eagerly with a HIP
-    # event pair around every kernel on the solver's stream (events cannot sit inside the
-    # replayed graph); kernel durations are the same either way, launch gaps are not.
-    prof_steps = min(args.steps, 20)
-    gpu.profile_enable(True)
-    gpu.profile_read()   # reset
-    for _ in range(prof_steps):
-        gpu.step(p, DT)
-    gpu.synchronize()
-    prof = gpu.profile_read()
-    gpu.profile_enable(False)
+    # Per-kernel durations of the same window: the saved state reloaded and the same steps
+    # launched eagerly with a HIP event pair around every kernel on the solver's stream
+    # (events cannot sit inside the replayed graph). Kernel durations are the same either
+    # way; launch gaps are not.
+    prof = profile_window(gpu, p, pos_w, vel_w, args.steps)
 
     value = n * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
@@ -346,9 +378,15 @@ def main():
         # it fills (committed rocprofv3 SQ/GRBM counters)
         "valu_busy": load_valu_busy(dom),
         "kernels_valu_busy": {k: load_valu_busy(k) for k in prof},
-        "profiled_steps": prof_steps,
+        "profiled_steps": args.steps,
+        "profiled_window": "the timed steps, replayed eagerly from the same saved state",
     }
 
+    # The same window in SPH_MODE_FAST_MATH (fused multiply-adds and hardware reciprocals,
+    # held to the 1e-5 tolerance instead of bit-exact), reported beside the headline.
+    fast = None
+    if not args.no_fast:
+        fast = fast_math_leg(n, p, pos_w, vel_w, args.steps, flags)
     # A renderer reads the particles back every frame (FluidSimulator::writeCpuInstances calls
     # particles()): the same steps, each followed by the 16-byte-per-particle readback to host
     # memory. Reported beside `value`, never as it.
@@ -394,6 +432,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "with_readback": readback,
+        "fast_math": fast,
         "surrogate": sur,
     }
     print(json.dumps(line), flush=True)

@@ -31,7 +31,7 @@ STATUS_NAMES = {
 
 SHAPE_SPHERE, SHAPE_BOX, SHAPE_PLANE, SHAPE_CYLINDER, SHAPE_CAPSULE, SHAPE_MESH = range(6)
 FIELD_WIND, FIELD_VORTEX, FIELD_POINT = range(3)
-MODE_REFERENCE, MODE_UNCAPPED, MODE_DEDUPE = 0, 1, 2
+MODE_REFERENCE, MODE_UNCAPPED, MODE_DEDUPE, MODE_FAST_MATH = 0, 1, 2, 4
 EOS_LINEAR, EOS_TAIT = 0, 1
 CFG_KEEP_FORCES = 1
 CFG_NO_GRAPH = 2

@@ -131,7 +131,7 @@ extern "C" int sph_read_forces(SphSolver* s, float* force4) {
 
 extern "C" int sph_set_mode(SphSolver* s, uint32_t modeFlags) {
     if (!s) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_set_mode: null solver");
-    if (modeFlags & ~(uint32_t)(SPH_MODE_UNCAPPED | SPH_MODE_DEDUPE)) return fail(SPH_ERR_INVALID_ARGUMENT, "unknown mode flag");
+    if (modeFlags & ~(uint32_t)(SPH_MODE_UNCAPPED | SPH_MODE_DEDUPE | SPH_MODE_FAST_MATH)) return fail(SPH_ERR_INVALID_ARGUMENT, "unknown mode flag");
     s->e.sc[0].mode = modeFlags;
     return SPH_OK;
 }
@@ -205,7 +205,7 @@ extern "C" int sph_batch_set_layout(SphBatch* b, uint32_t scene, const uint32_t 
 extern "C" int sph_batch_set_mode(SphBatch* b, uint32_t scene, uint32_t modeFlags, uint32_t eos) {
     sphhost::clear_error();
     if (!b || scene >= b->e.sc.size()) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_batch_set_mode: bad argument");
-    if (modeFlags & ~(uint32_t)(SPH_MODE_UNCAPPED | SPH_MODE_DEDUPE)) return fail(SPH_ERR_INVALID_ARGUMENT, "unknown mode flag");
+    if (modeFlags & ~(uint32_t)(SPH_MODE_UNCAPPED | SPH_MODE_DEDUPE | SPH_MODE_FAST_MATH)) return fail(SPH_ERR_INVALID_ARGUMENT, "unknown mode flag");
     if (eos != SPH_EOS_LINEAR && eos != SPH_EOS_TAIT) return fail(SPH_ERR_INVALID_ARGUMENT, "unknown eos");
     b->e.sc[scene].mode = modeFlags;
     b->e.sc[scene].eos = eos;

@@ -121,8 +121,11 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // buckets: it keeps no bucket hashes live, which leaves the reference-mode pass at 64 VGPRs
 // (8 waves per SIMD, 4 VGPRs spilled, measured faster than one wave fewer) and the pass
 // with sums at 94 (5 waves, which its 32 KB of LDS per workgroup also allows).
-template <bool SUMS, bool DEDUPE>
-__global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
+// FAST (instantiated without SUMS only): scenes with SPH_MODE_FAST_MATH evaluate r2 with
+// fused multiply-adds and poly6 with the mass folded into its constant; other scenes of the
+// launch keep the reference's arithmetic.
+template <bool SUMS, bool DEDUPE, bool FAST = false>
+__global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) : 1)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
               const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
               float4* __restrict__ sums, float2* __restrict__ deviation) {
@@ -160,10 +163,19 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : 8) : 1)
         invH2 = 1.0f / k.h2;
         invH = 1.0f / k.h;
     }
+    const bool fast = FAST && (k.mode & SPH_MODE_FAST_MATH) != 0;
+    const float massPoly6 = k.mass * k.cPoly6;
     // One candidate at slot u, in visiting order; `valid` is false for the padding reads.
     // Returns whether it is in range.
     auto visit = [&](bool valid, uint32_t u, float qx, float qy, float qz) -> bool {
         const float ddx = px - qx, ddy = py - qy, ddz = pz - qz;
+        if (FAST && fast) {
+            const float r2f = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+            const bool inrf = valid && r2f < k.h2;
+            const float df = k.h2 - r2f;
+            density = inrf ? fmaf(massPoly6, (df * df) * df, density) : density;
+            return inrf;
+        }
         const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
         // poly6 is 0 at r2 >= h2 and density + mass * 0 == density: such candidates leave
         // the sum as it is.
@@ -756,15 +768,49 @@ __device__ __forceinline__ void pair_force_from(const StepConsts& k, const float
     f[2] += ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
 }
 
-__device__ __forceinline__ void pair_force(const StepConsts& k, const float4* __restrict__ pos,
+// The same contribution for scenes with SPH_MODE_FAST_MATH: r2 by fused multiply-adds (the
+// expression the fast density pass uses, so its neighbour list stays a superset), 1 / r from
+// the hardware reciprocal square root, 1 / rho_j from the hardware reciprocal, products
+// accumulated with fused multiply-adds. Not bit-exact: within the 1e-5 tolerance (relative to
+// the largest force) of the reference after one substep.
+__device__ __forceinline__ void pair_force_fast(const StepConsts& k, const float4 q, const float4 ov, float px,
+                                                float py, float pz, float vx, float vy, float vz, float pressure,
+                                                float f[3]) {
+    const float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
+    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+    if (r2 >= k.r2Cut || !(r2 > 0.0f)) return;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float distance = r2 * inv;
+    if (distance >= k.h || distance <= 1e-6f) return;
+    const float otherDensity = q.w < 1e-6f ? 1e-6f : q.w;
+    const float irho = __builtin_amdgcn_rcpf(otherDensity);
+    const float d = k.h - distance;
+    // -m (p_i + p_j) / (2 rho_j) * cSpiky d^2 / r * delta
+    const float sp = ((k.negMass * (pressure + ov.w)) * (0.5f * irho)) * (((k.cSpiky * d) * d) * inv);
+    f[0] = fmaf(sp, ddx, f[0]);
+    f[1] = fmaf(sp, ddy, f[1]);
+    f[2] = fmaf(sp, ddz, f[2]);
+    // mu m (v_j - v_i) / rho_j * cVisc (h - r)
+    const float vl = (k.viscMass * irho) * (k.cVisc * d);
+    f[0] = fmaf(vl, ov.x - vx, f[0]);
+    f[1] = fmaf(vl, ov.y - vy, f[1]);
+    f[2] = fmaf(vl, ov.z - vz, f[2]);
+}
+
+template <bool FAST>
+__device__ __forceinline__ void pair_force(const StepConsts& k, bool fast, const float4* __restrict__ pos,
                                            const float4* __restrict__ vel, uint32_t u, float px, float py, float pz,
                                            float vx, float vy, float vz, float pressure, float f[3]) {
-    pair_force_from(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
+    if (FAST && fast)
+        pair_force_fast(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
+    else
+        pair_force_from(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
 }
 
 // 64 VGPRs at most (8 waves per SIMD): this VALU-bound pass runs ~4 % faster at 8 waves
 // than at the 6 its natural register count allows, with no spills.
-__global__ void __launch_bounds__(kBlock, 8)
+template <bool FAST>
+__global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
     k_force(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
             const uint32_t* __restrict__ bucketStart, const uint32_t* __restrict__ counts,
             const uint4* __restrict__ nbrList, const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut,
@@ -786,9 +832,29 @@ __global__ void __launch_bounds__(kBlock, 8)
     const float density = self.w < 1e-6f ? 1e-6f : self.w;
     float vx = selfVel.x, vy = selfVel.y, vz = selfVel.z;
     const float pressure = selfVel.w;
+    const bool fast = FAST && (k.mode & SPH_MODE_FAST_MATH) != 0;
     float f[3] = {0.0f, 0.0f, 0.0f};
     const uint32_t listed = nbrCount ? nbrCount[t] : k.listCap + 1u;
-    if (listed <= k.listCap) {
+    if (FAST && fast && listed <= k.listCap) {
+        // Fast arithmetic leaves this loop bound by memory latency: eight pairs in flight.
+        for (uint32_t e = 0; e < listed; e += 8) {
+            const uint4 g0 = load_stream(nbrList + (size_t)(e >> 2) * L.total + t);
+            const uint4 g1 = e + 4 < listed ? load_stream(nbrList + (size_t)((e >> 2) + 1) * L.total + t) : g0;
+            const uint32_t us[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+            float4 q[8], ov[8];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r) {
+                const uint32_t u = e + r < listed ? us[r] : us[0];
+                q[r] = pos[u];
+                ov[r] = vel[u];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r) {
+                if (e + r < listed && us[r] != t)
+                    pair_force_fast(k, q[r], ov[r], px, py, pz, vx, vy, vz, pressure, f);
+            }
+        }
+    } else if (listed <= k.listCap) {
         // The in-range candidates recorded by the density pass, in visiting order.
         for (uint32_t e = 0; e < listed; e += 4) {
             const uint4 g = load_stream(nbrList + (size_t)(e >> 2) * L.total + t);
@@ -826,7 +892,7 @@ __global__ void __launch_bounds__(kBlock, 8)
                     for (uint32_t j = 0; j < cnt; ++j) {
                         const uint32_t u = lo + j;
                         if (u == t) continue;
-                        pair_force(k, pos, vel, u, px, py, pz, vx, vy, vz, pressure, f);
+                        pair_force<FAST>(k, fast, pos, vel, u, px, py, pz, vx, vy, vz, pressure, f);
                     }
                 }
             }
@@ -895,14 +961,21 @@ hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint3
         else
             k_density<true, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nullptr, nullptr,
                                                                sums->sums, sums->deviation);
-    } else if (tiles && !dedupe && L.tiles > 0 && L.tileScene) {
+    } else if (tiles && !dedupe && !(L.modes & SPH_MODE_FAST_MATH) && L.tiles > 0 && L.tileScene) {
         k_density_tiled<<<L.tiles + tiles->restBlocks, kBlock, 0, s>>>(L, tiles->binStart, pos, vel,
                                                                        bucketStart, counts, nbrList, nbrCount,
                                                                        tiles->rest, tiles->restCount);
     } else {
-        if (dedupe)
+        const bool fast = (L.modes & SPH_MODE_FAST_MATH) != 0;
+        if (dedupe && fast)
+            k_density<false, true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList,
+                                                                     nbrCount, nullptr, nullptr);
+        else if (dedupe)
             k_density<false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
                                                                nullptr, nullptr);
+        else if (fast)
+            k_density<false, false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList,
+                                                                      nbrCount, nullptr, nullptr);
         else
             k_density<false, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
                                                                 nullptr, nullptr);
@@ -914,7 +987,12 @@ hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, c
                         const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_force<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut, forceOut);
+    if (L.modes & SPH_MODE_FAST_MATH)
+        k_force<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut,
+                                                  forceOut);
+    else
+        k_force<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut,
+                                                   forceOut);
     return hipGetLastError();
 }
 

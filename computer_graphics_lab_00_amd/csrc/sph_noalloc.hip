@@ -110,7 +110,7 @@ extern "C" int sph_substep_device(const SphParams* p, float dt, uint32_t modeFla
         return sphhost::fail(SPH_ERR_CAPACITY, "sph_substep_device: cellCount above SPH_NOALLOC_MAX_CELLS (" +
                                                    std::to_string(kMaxC) + ")");
     if (!(p->smoothingRadius > 0.0f)) return sphhost::fail(SPH_ERR_INVALID_ARGUMENT, "smoothingRadius must be > 0");
-    if (modeFlags & ~(uint32_t)(SPH_MODE_UNCAPPED | SPH_MODE_DEDUPE))
+    if (modeFlags & ~(uint32_t)(SPH_MODE_UNCAPPED | SPH_MODE_DEDUPE | SPH_MODE_FAST_MATH))
         return sphhost::fail(SPH_ERR_INVALID_ARGUMENT, "sph_substep_device: unknown mode flags");
     if (eos != SPH_EOS_LINEAR && eos != SPH_EOS_TAIT)
         return sphhost::fail(SPH_ERR_INVALID_ARGUMENT, "sph_substep_device: unknown equation of state");

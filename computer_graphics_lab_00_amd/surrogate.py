@@ -434,7 +434,8 @@ def _energy(pos, vel, params: SphParams) -> float:
 
 
 def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 / 600.0,
-            threshold: float = DEFAULT_THRESHOLD, stats_every: int = 1, unchecked: bool = True, log=None) -> dict:
+            threshold: float = DEFAULT_THRESHOLD, stats_every: int = 1, unchecked: bool = True, log=None,
+            progress_every: int = 20) -> dict:
     """Runs `steps` FluidSolver.step(dt) calls of `workload` with the plain path and with
     the surrogate from the same initial state and returns timing and density/energy
     statistics. The timed runs carry no statistics work; the statistics come from
@@ -448,7 +449,11 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     solver = FluidSolver(n, workload.cell_count)
     result = {"particles": n, "steps": steps, "substeps_per_step": sub, "threshold": threshold}
 
-    def run(use_net: bool, timed: bool):
+    def say(msg):
+        if log is not None:
+            print(f"[surrogate] {msg}", file=log, flush=True)
+
+    def run(use_net: bool, timed: bool, label: str):
         nonlocal threshold
         solver.load_state(pos0, vel0)
         net.set_backoff(net.max_backoff)   # every run starts with no back-off pending
@@ -463,31 +468,32 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
                 solver.step(p, dt)
             if not timed and (i + 1) % stats_every == 0:
                 series.append(density_deviation(solver, net, p, dt_sub))
+            # progress of the untimed runs (the host waits for the device here, so a slow run
+            # shows its pace instead of going quiet); timed runs report once at their end
+            if not timed and (i + 1) % progress_every == 0:
+                solver.synchronize()
+                say(f"{label}: step {i + 1}/{steps}, {time.perf_counter() - t0:.1f} s")
         solver.synchronize()
         return time.perf_counter() - t0, st, series
 
-    def say(msg):
-        if log is not None:
-            print(f"[surrogate] {msg}", file=log, flush=True)
-
-    run(False, True)   # warm-up: plans, graphs
+    run(False, False, "warm-up (plain)")   # plans, graphs; untimed, so it reports progress
     say("warm-up done")
-    t_plain, _, _ = run(False, True)
+    t_plain, _, _ = run(False, True, "plain")
     say(f"plain: {1e3 * t_plain / steps:.3f} ms/step")
     net.inference_time()
-    t_sur, st, _ = run(True, True)
+    t_sur, st, _ = run(True, True, "surrogate")
     say(f"surrogate: {1e3 * t_sur / steps:.3f} ms/step")
     mlp_ms, mlp_rows = net.inference_time()
     if unchecked:
         # the same rollout with the check never failing: the cost of the surrogate alone
         saved = threshold
         threshold = float("inf")
-        t_free, _, _ = run(True, True)
+        t_free, _, _ = run(True, True, "surrogate, check off")
         say(f"surrogate, check off: {1e3 * t_free / steps:.3f} ms/step")
         net.inference_time()
         result["unchecked_ms_per_step"] = 1e3 * t_free / steps
         result["unchecked_speedup"] = t_plain / t_free
-        _, _, free_series = run(True, False)
+        _, _, free_series = run(True, False, "surrogate statistics, check off")
         result["unchecked_density_excess"] = [float(v[0]) for v in free_series]
         threshold = saved
     result["plain_ms_per_step"] = 1e3 * t_plain / steps
@@ -512,9 +518,9 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
         print(f"[surrogate] timed: plain {result['plain_ms_per_step']:.3f} ms/step, surrogate "
               f"{result['surrogate_ms_per_step']:.3f} ms/step, fallbacks {st.fallbacks}/{st.macroSteps}",
               file=log, flush=True)
-    _, _, plain_series = run(False, False)
+    _, _, plain_series = run(False, False, "plain statistics")
     e_plain = _energy(*solver.state(), p)
-    _, st2, sur_series = run(True, False)
+    _, st2, sur_series = run(True, False, "surrogate statistics")
     e_sur = _energy(*solver.state(), p)
     e0 = _energy(pos0, vel0, p)
     ps = np.array(plain_series)

@@ -435,6 +435,47 @@ def test_m1_developed_state_one_substep(cells, oracle_lib):
     gpu.close()
 
 
+@pytest.mark.parametrize("case", ["s16k", "m1", "m1_developed"])
+def test_fast_math_mode_within_tolerance(case, oracle_lib):
+    """SPH_MODE_FAST_MATH: the same grid (bit-exact) and the same neighbour semantics, the
+    arithmetic by fused multiply-adds and hardware reciprocals. After one substep the
+    density is within 1e-5 relative and the force within 1e-5 of the largest force of the
+    reference (the FP32 oracle); both errors against the FP64 oracle are printed beside the
+    reference's own."""
+    say(f"case: fast math, {case}")
+    w = fluid.s16k() if case == "s16k" else fluid.m1()
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    if case == "s16k":
+        rng = np.random.default_rng(2)
+        vel[:, :3] = rng.normal(0, 0.5, (w.count, 3)).astype(np.float32)
+    if case == "m1_developed":
+        with fluid.FluidSolver(w.count, p.cellCount) as run:
+            run.load_state(pos, vel)
+            for _ in range(60):
+                run.step(p, 1.0 / 600.0)
+            pos, vel = run.state()
+    dt = 1.0 / 600.0 / fluid.fluid_substeps(p, 1.0 / 600.0)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib, mode=capi.MODE_FAST_MATH)
+    g_cell, g_counts, g_slots = gpu.grid(p.cellCount)
+    o_cell, o_counts, o_slots = ora.grid(p.cellCount)
+    assert np.array_equal(g_cell, o_cell) and np.array_equal(g_counts, o_counts)
+    gp, _ = gpu.state()
+    op, _ = ora.state()
+    e_rho = rel_err(gp[:, 3], op[:, 3])
+    e_f = norm_err(gpu.forces()[:, :3], ora.forces()[:, :3])
+    say(f"  fast vs reference: density max rel err {e_rho:.2e}, force max err / max|F| {e_f:.2e}")
+    fp64_report(p, pos, vel, dt, gpu, oracle_lib, f"fast math {case}")
+    o64 = oracle_lib.OracleSolver(64)
+    o64.load_state(pos, vel)
+    o64.substep(p, dt, THREADS)
+    say(f"  reference (FP32 oracle) vs FP64: density {rel_err(op[:, 3], o64.state()[0][:, 3]):.2e}, "
+        f"force {norm_err(ora.forces()[:, :3], o64.forces()[:, :3]):.2e}")
+    assert e_rho <= REL_TOL, "density beyond 1e-5"
+    assert e_f <= REL_TOL, "force beyond 1e-5"
+    gpu.close()
+
+
 def test_m1_full_run_properties():
     """Size-independent properties at the full size after 5 steps (40 substeps): the
     sorted layout is a permutation in ascending-index buckets, mass is conserved, the
