@@ -71,11 +71,13 @@ def bench_batch64(args) -> None:
     from computer_graphics_lab_00_amd import batch
 
     specs = batch.batch64(2024)
+    schedule = None
     if args.tune or TUNE_CACHE.exists():
         print("[bench] autotune (cache: %s)" % TUNE_CACHE, file=sys.stderr, flush=True)
         specs = batch.autotune(specs, TUNE_CACHE, retune=args.tune)
+        schedule = batch.autotune_schedule(specs, TUNE_CACHE, retune=args.tune)
     total = sum(s.count for s in specs)
-    with batch.SceneBatch(specs) as b:
+    with batch.SceneBatch(specs, schedule=schedule) as b:
         b.load_initial()
         for _ in range(args.warmup):
             b.step(DT)
@@ -116,7 +118,10 @@ def bench_batch64(args) -> None:
         "data": "synthetic",
         "config": {"workload": "batch64", "scenes": len(specs), "particles": total,
                    "counts": [s.count for s in specs], "seed": 2024, "dt": DT,
-                   "tuned": TUNE_CACHE.exists()},
+                   "tuned": TUNE_CACHE.exists(),
+                   "block_schedule": "scene order" if schedule is None else
+                   ("largest first" if schedule == batch.schedule_order(specs, "largest first") else
+                    "smallest first")},
         "sequential_ms_per_step": seq * 1e3 / args.steps,
         "batched_speedup_vs_sequential": seq / elapsed,
     }

@@ -103,7 +103,7 @@ def batch64(seed: int = 2024, scenes: int = 64, n_min: int = 2000, n_max: int = 
 class SceneBatch:
     """Independent scenes advanced together (sph_batch_*)."""
 
-    def __init__(self, specs: list[SceneSpec], device: int = 0, flags: int = 0):
+    def __init__(self, specs: list[SceneSpec], device: int = 0, flags: int = 0, schedule: list[int] | None = None):
         self._lib = capi.load()
         self.specs = specs
         caps = (SphSceneCapacity * len(specs))(*[SphSceneCapacity(s.count, s.cells()) for s in specs])
@@ -117,6 +117,9 @@ class SceneBatch:
             check(self._lib.sph_batch_set_params(self._h, i, C.byref(p)))
             tile = (C.c_uint32 * 3)(*s.tile)
             check(self._lib.sph_batch_set_layout(self._h, i, tile, 1 if s.neighbour_list else 0))
+        if schedule:
+            order = (C.c_uint32 * len(schedule))(*schedule)
+            check(self._lib.sph_batch_set_schedule(self._h, order, len(schedule)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -206,9 +209,53 @@ def _time_scene(spec: SceneSpec, dt: float, steps: int, warmup: int) -> float:
         return (time.perf_counter() - t0) * 1e3 / steps
 
 
+SCHEDULES = ("scene order", "largest first", "smallest first")
+
+
+def schedule_order(specs: list[SceneSpec], name: str) -> list[int] | None:
+    """The block schedule `name` as a scene order (None: scene order)."""
+    if name == "largest first":
+        return sorted(range(len(specs)), key=lambda i: (-specs[i].count, i))
+    if name == "smallest first":
+        return sorted(range(len(specs)), key=lambda i: (specs[i].count, i))
+    return None
+
+
+def _time_batch(specs: list[SceneSpec], schedule: list[int] | None, dt: float, steps: int, warmup: int) -> float:
+    with SceneBatch(specs, schedule=schedule) as b:
+        b.load_initial()
+        for _ in range(warmup):
+            b.step(dt)
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.step(dt)
+        b.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def autotune_schedule(specs: list[SceneSpec], cache_path: str | Path, dt: float = 1.0 / 600.0, steps: int = 10,
+                      warmup: int = 3, retune: bool = False) -> list[int] | None:
+    """Choose the batch's block schedule (the order of the scenes' workgroups in every
+    launch) by timing the whole batch with each of SCHEDULES; cached by the batch's
+    signature. Returns the scene order (None: scene order)."""
+    cache_path = Path(cache_path)
+    cache = json.loads(cache_path.read_text()) if cache_path.exists() else {}
+    key = "batch:" + "|".join(f"{s.signature()}:{s.cell_count}:{s.tile}:{s.neighbour_list}" for s in specs)
+    if key not in cache or retune:
+        timed = {name: _time_batch(specs, schedule_order(specs, name), dt, steps, warmup) for name in SCHEDULES}
+        best = min(timed, key=timed.get)
+        cache[key] = {"schedule": best, "ms_per_step": timed}
+        print(f"[autotune] block schedule: {best} ({timed})", file=sys.stderr, flush=True)
+        cache_path.parent.mkdir(parents=True, exist_ok=True)
+        cache_path.write_text(json.dumps(cache, indent=1, sort_keys=True) + "\n")
+    return schedule_order(specs, cache[key]["schedule"])
+
+
 def autotune(specs: list[SceneSpec], cache_path: str | Path, dt: float = 1.0 / 600.0, steps: int = 6,
              warmup: int = 2, retune: bool = False) -> list[SceneSpec]:
-    """Choose cellCount, tile and list/bucket-walk per scene. Cached by scene signature in
+    """Choose cellCount, tile and list/bucket-walk per scene (each timed alone; the batch's
+    block schedule is chosen by autotune_schedule). Cached by scene signature in
     `cache_path`; with a cache hit nothing is timed and the cached choice is returned."""
     cache_path = Path(cache_path)
     cache = json.loads(cache_path.read_text()) if cache_path.exists() else {}

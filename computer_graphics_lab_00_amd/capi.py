@@ -221,6 +221,7 @@ def _declare(lib):
         "sph_batch_set_params": (C.c_int, [P, C.c_uint32, C.POINTER(SphParams)]),
         "sph_batch_set_layout": (C.c_int, [P, C.c_uint32, _UP, C.c_uint32]),
         "sph_batch_set_mode": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "sph_batch_set_schedule": (C.c_int, [P, _UP, C.c_uint32]),
         "sph_batch_emit": (C.c_int, [P, C.c_uint32, C.c_uint32]),
         "sph_batch_load_state": (C.c_int, [P, C.c_uint32, _FP, _FP, C.c_uint32]),
         "sph_batch_step": (C.c_int, [P, C.c_float]),

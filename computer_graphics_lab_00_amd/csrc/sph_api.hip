@@ -202,6 +202,12 @@ extern "C" int sph_batch_set_layout(SphBatch* b, uint32_t scene, const uint32_t 
     return SPH_OK;
 }
 
+extern "C" int sph_batch_set_schedule(SphBatch* b, const uint32_t* order, uint32_t count) {
+    sphhost::clear_error();
+    if (!b) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_batch_set_schedule: null batch");
+    return b->e.set_schedule(order, count);
+}
+
 extern "C" int sph_batch_set_mode(SphBatch* b, uint32_t scene, uint32_t modeFlags, uint32_t eos) {
     sphhost::clear_error();
     if (!b || scene >= b->e.sc.size()) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_batch_set_mode: bad argument");

@@ -53,6 +53,9 @@ public:
     int emit(uint32_t scene, const SphParams& p, uint32_t count);
     int load_state(uint32_t scene, const float* pos4, const float* vel4, uint32_t count);
     int set_params(uint32_t scene, const SphParams& p);
+    // Order of the scenes' workgroups in every batched launch (a permutation of the scenes;
+    // count 0 restores scene order). Results do not depend on it.
+    int set_schedule(const uint32_t* order, uint32_t count);
     // Every scene advances by dt with its own substep rule (FluidSolver::step); when
     // `singleSubstep` the scenes advance one substep of exactly dt.
     int step(float dt, bool singleSubstep);
@@ -103,6 +106,7 @@ public:
     uint32_t planned_blocks() const { return planBlocks; }
 
     std::vector<SceneHost> sc;
+    std::vector<uint32_t> schedule;   // empty: scene order
     int device = 0;
     uint32_t flags = 0;
     bool profiling = false;

@@ -330,6 +330,22 @@ void Engine::invalidate() {
     }
 }
 
+int Engine::set_schedule(const uint32_t* order, uint32_t count) {
+    if (count == 0) {
+        schedule.clear();
+        return SPH_OK;
+    }
+    if (!order || count != sc.size()) return fail(SPH_ERR_INVALID_ARGUMENT, "schedule: one entry per scene");
+    std::vector<uint8_t> seen(sc.size(), 0);
+    for (uint32_t q = 0; q < count; ++q) {
+        if (order[q] >= sc.size() || seen[order[q]]) return fail(SPH_ERR_INVALID_ARGUMENT, "schedule: not a permutation");
+        seen[order[q]] = 1;
+    }
+    schedule.assign(order, order + count);
+    touch();
+    return SPH_OK;
+}
+
 int Engine::set_params(uint32_t scene, const SphParams& p) {
     if (scene >= sc.size()) return fail(SPH_ERR_INVALID_ARGUMENT, "scene index out of range");
     if (!is_pow2(p.cellCount)) return fail(SPH_ERR_NOT_POWER_OF_TWO, "cellCount must be a power of two");
@@ -437,6 +453,7 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
         put(&s.list, sizeof(s.list));
         if (s.n) put(&s.params, sizeof(s.params));
     }
+    put(schedule.data(), schedule.size() * sizeof(uint32_t));
     if (key == planKey) return SPH_OK;
     invalidate();
     // Tables into pinned memory, then one copy each (the previous upload has finished:
@@ -447,16 +464,27 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
     sphdev::BoundaryConsts* bounds = reinterpret_cast<sphdev::BoundaryConsts*>(h + kMaxSubsteps * S * sizeof(sphdev::SceneDesc));
     uint16_t* blockScene = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(bounds) + S * sizeof(sphdev::BoundaryConsts));
     uint16_t* tileScene = blockScene + (maxBlocks + 1);
+    // Workgroups of particle-parallel (and half-tile) launches are laid out scene by scene
+    // in the order of `schedule` (scene order unless set): offsets first, tables after.
+    std::vector<uint32_t> blockOff(S, 0), tileOff(S, 0);
     uint32_t blocks = 0, tiles = 0;
+    for (size_t q = 0; q < S; ++q) {
+        const uint32_t i = schedule.empty() ? (uint32_t)q : schedule[q];
+        const SceneHost& s = sc[i];
+        const uint32_t b = blocks_for(s.n);
+        const uint32_t sceneTiles = s.n ? make_bins(s, s.params).bins / sphdev::kHalfTileBins : 0u;
+        blockOff[i] = blocks;
+        tileOff[i] = tiles;
+        for (uint32_t r = 0; r < b; ++r) blockScene[blocks + r] = (uint16_t)i;
+        for (uint32_t r = 0; r < sceneTiles; ++r) tileScene[tiles + r] = (uint16_t)i;
+        blocks += b;
+        tiles += sceneTiles;
+    }
     uint32_t binBase = 0;   // bins are packed by the bins each scene actually uses
     for (size_t i = 0; i < S; ++i) {
         const SceneHost& s = sc[i];
-        const uint32_t b = blocks_for(s.n);
         const sphdev::BinGrid grid = s.n ? make_bins(s, s.params) : sphdev::BinGrid{};
         const uint32_t sceneBins = s.n ? grid.bins : 0u;
-        for (uint32_t q = 0; q < b; ++q) blockScene[blocks + q] = (uint16_t)i;
-        const uint32_t sceneTiles = sceneBins / sphdev::kHalfTileBins;   // bins come in tiles of 64
-        for (uint32_t q = 0; q < sceneTiles; ++q) tileScene[tiles + q] = (uint16_t)i;
         sphdev::BoundaryConsts& bc = bounds[i];
         memset(&bc, 0, sizeof(bc));
         if (s.n) {
@@ -471,8 +499,8 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
             d.cellOffset = s.cellOffset;
             d.binOffset = binBase;
             d.binCapacity = sceneBins;
-            d.blockOffset = blocks;
-            d.tileOffset = tiles;
+            d.blockOffset = blockOff[i];
+            d.tileOffset = tileOff[i];
             d.active = j < sub[i] ? 1u : 0u;
             if (s.n) {
                 d.k = make_consts(s, s.params, dts[i], nbrList != nullptr);
@@ -486,8 +514,6 @@ int Engine::ensure_plan(float dt, bool singleSubstep, uint32_t* maxSubOut) {
             }
             d.k.n = s.n;
         }
-        blocks += b;
-        tiles += sceneTiles;
         binBase += sceneBins + 1u;
     }
     planBlocks = blocks;

@@ -297,6 +297,10 @@ int sph_batch_set_params(SphBatch* batch, uint32_t scene, const SphParams* param
  * neighbour list. */
 int sph_batch_set_layout(SphBatch* batch, uint32_t scene, const uint32_t tileLog2[3], uint32_t neighbourList);
 int sph_batch_set_mode(SphBatch* batch, uint32_t scene, uint32_t modeFlags, uint32_t eos);
+/* Block schedule: the order in which the scenes' workgroups are laid out in every batched
+ * launch (`order` is a permutation of the scene indices; count 0 restores scene order).
+ * A layout knob for the autotuner: results do not depend on it. */
+int sph_batch_set_schedule(SphBatch* batch, const uint32_t* order, uint32_t count);
 int sph_batch_emit(SphBatch* batch, uint32_t scene, uint32_t count);
 int sph_batch_load_state(SphBatch* batch, uint32_t scene, const float* pos4, const float* vel4, uint32_t count);
 /* FluidSolver::step for every scene; dt <= 0 is a no-op. Enqueued, returns early. */

@@ -130,3 +130,47 @@ def test_autotune_then_cache(tmp_path):
         b.load_initial()
         b.step(DT)
         assert np.isfinite(b.state(0)[0]).all()
+
+
+def test_block_schedule_does_not_change_results():
+    """The block schedule only reorders workgroups: largest-first equals scene order bit
+    for bit."""
+    specs = small_specs()
+    outs = []
+    for name in ("scene order", "largest first"):
+        with batch.SceneBatch(specs, schedule=batch.schedule_order(specs, name)) as b:
+            b.load_initial()
+            for _ in range(3):
+                b.step(DT)
+            outs.append([b.state(i) for i in range(len(specs))])
+    for i in range(len(specs)):
+        assert np.array_equal(bits(outs[0][i][0]), bits(outs[1][i][0]))
+        assert np.array_equal(bits(outs[0][i][1]), bits(outs[1][i][1]))
+
+
+def test_batch64_at_config_size(oracle_lib):
+    """The batched workload at its configured size: 64 scenes of 2k-200k particles (2.29M),
+    one step(1/600) of every scene in one graph replay. The smallest, the median and the
+    largest scene equal the same scene stepped alone and the oracle's step, bit for bit."""
+    specs = batch.batch64(2024)
+    total = sum(s.count for s in specs)
+    say(f"case: batch64, {len(specs)} scenes, {total} particles")
+    assert len(specs) == 64 and total > 2_000_000
+    order = sorted(range(len(specs)), key=lambda i: specs[i].count)
+    picks = [order[0], order[len(order) // 2], order[-1]]
+    with batch.SceneBatch(specs) as b:
+        b.load_initial()
+        b.step(DT)
+        got = {i: b.state(i) for i in picks}
+    for i in picks:
+        s = specs[i]
+        alone = run_alone(s, 1)
+        p = s.params()
+        pos, vel = s.initial_state(p)
+        ora = oracle_lib.OracleSolver(32)
+        ora.load_state(pos, vel)
+        ora.step(p, DT, 16)
+        op, ov = ora.state()
+        say(f"  scene {i}: {s.count} particles, {fluid.fluid_substeps(p, DT)} substeps")
+        assert np.array_equal(bits(got[i][0]), bits(alone[0])) and np.array_equal(bits(got[i][1]), bits(alone[1]))
+        assert np.array_equal(bits(got[i][0]), bits(op)) and np.array_equal(bits(got[i][1]), bits(ov))
