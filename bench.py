@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--workload", default="dam_break_1m", choices=["dam_break_1m", "batch64"],
                     help="dam_break_1m: the headline line; batch64: 64 independent scenes in one hipGraph")
     ap.add_argument("--tune", action="store_true", help="batch64: re-run the autotuner instead of its cache")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch64: skip the comparison with the scenes stepped one after another")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-math leg")
     ap.add_argument("--surrogate-steps", type=int, default=30)
@@ -91,7 +93,7 @@ def bench_batch64(args) -> None:
           file=sys.stderr, flush=True)
     # The same scenes one after another, each in its own single-scene batch.
     seq = 0.0
-    for s in specs:
+    for s in ([] if args.no_sequential else specs):
         with batch.SceneBatch([s]) as one:
             one.load_initial()
             for _ in range(2):
@@ -122,8 +124,8 @@ def bench_batch64(args) -> None:
                    "block_schedule": "scene order" if schedule is None else
                    ("largest first" if schedule == batch.schedule_order(specs, "largest first") else
                     "smallest first")},
-        "sequential_ms_per_step": seq * 1e3 / args.steps,
-        "batched_speedup_vs_sequential": seq / elapsed,
+        "sequential_ms_per_step": None if args.no_sequential else seq * 1e3 / args.steps,
+        "batched_speedup_vs_sequential": None if args.no_sequential else seq / elapsed,
     }
     print(json.dumps(line), flush=True)
 
