@@ -37,6 +37,7 @@ CFG_KEEP_FORCES = 1
 CFG_NO_GRAPH = 2
 CFG_BUCKET_WALK = 4
 CFG_TILED_DENSITY = 8
+CHECK_COMPRESSION, CHECK_RAREFACTION = 1, 2
 
 
 class SphCollider(C.Structure):
@@ -166,6 +167,9 @@ class SphRolloutStats(C.Structure):
         ("backedOff", C.c_uint64),
         ("maxDeviation", C.c_float),
         ("lastDeviation", C.c_float),
+        ("maxRarefactionGrowth", C.c_float),
+        ("pad0", C.c_uint32),
+        ("rarefactionFallbacks", C.c_uint64),
     ]
 
 
@@ -264,6 +268,7 @@ def _declare(lib):
         "sph_net_inference_time": (C.c_int, [P, _DP, C.POINTER(C.c_uint64)]),
         "sph_net_set_smoothing": (C.c_int, [P, C.c_float]),
         "sph_net_set_backoff": (C.c_int, [P, C.c_uint32]),
+        "sph_net_set_check": (C.c_int, [P, C.c_uint32]),
         "sph_density_deviation": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, _DP]),
         "sph_substep_device": (C.c_int, [C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

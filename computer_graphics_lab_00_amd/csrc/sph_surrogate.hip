@@ -87,6 +87,11 @@ struct SphNet {
     // state carry its substep length (dt and the speed clamp of the integrate): `preparedDt`.
     uint64_t preparedEpoch = 0, predictedEpoch = 0;
     float preparedDt = 0.0f;
+    // Density statistics of the state of epoch `checkEpoch` (mean excess, mean |deviation|):
+    // the previous check's, which describe the next macro-step's starting state.
+    uint64_t checkEpoch = 0;
+    double checkDev[2] = {0.0, 0.0};
+    uint32_t checkSides = 3;   // SPH_CHECK_COMPRESSION | SPH_CHECK_RAREFACTION
     // The inference buffers (sums, ix, iy, keep*, the check's) are used on the stream of the
     // solver a call works with; a call on another solver's stream first waits for `done`,
     // recorded at the end of the previous call on `lastStream`.
@@ -813,6 +818,15 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
                 net->preparedEpoch = e.epoch();
                 net->preparedDt = dtSub;
             }
+            // The starting state's density statistics, for the rarefaction side of the check:
+            // the previous check's when it described this state, else reduced now from the
+            // prepare's workgroup sums (after a fallback or on a fresh state).
+            if ((net->checkSides & SPH_CHECK_RAREFACTION) && net->checkEpoch != e.epoch()) {
+                if (int rc = launch_check(net, e, true)) return rc;
+                if (int rc = finish_check(net, e, net->checkDev)) return rc;
+                net->checkEpoch = e.epoch();
+            }
+            const double deficitBefore = net->checkDev[1] - net->checkDev[0];
             if (net->predictedEpoch != e.epoch())
                 if (int rc = predict(net, e, 1u - net->predictedSlot)) return rc;
             const uint32_t used = net->predictedSlot;
@@ -839,7 +853,19 @@ extern "C" int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* p, f
             st.macroSteps += 1;
             st.lastDeviation = (float)dev[0];
             if ((float)dev[0] > st.maxDeviation) st.maxDeviation = (float)dev[0];
-            if (dev[0] > (double)threshold) {
+            // two-sided: compression (mean excess above the threshold) or rarefaction (the mean
+            // deficit grew by more than the threshold over the macro-step)
+            const double growth = (dev[1] - dev[0]) - deficitBefore;
+            if ((float)growth > st.maxRarefactionGrowth) st.maxRarefactionGrowth = (float)growth;
+            const bool compressed = (net->checkSides & SPH_CHECK_COMPRESSION) && dev[0] > (double)threshold;
+            const bool rarefied = (net->checkSides & SPH_CHECK_RAREFACTION) && growth > (double)threshold;
+            if (rarefied && !compressed) st.rarefactionFallbacks += 1;
+            if (!compressed && !rarefied) {
+                net->checkEpoch = e.epoch();   // these statistics describe the next starting state
+                net->checkDev[0] = dev[0];
+                net->checkDev[1] = dev[1];
+            }
+            if (compressed || rarefied) {
                 // The check failed: back to the kept state, and the plain path redoes the
                 // macro-step. The kept state is in the prepared (sorted) order, which is the
                 // order of the committed ids.
@@ -873,6 +899,14 @@ extern "C" int sph_net_set_backoff(SphNet* n, uint32_t maxBackoff) {
     n->maxBackoff = maxBackoff;
     n->backoff = 0;
     n->skip = 0;
+    return SPH_OK;
+}
+
+// Which sides of the density check sph_rollout applies (SPH_CHECK_*; both by default).
+extern "C" int sph_net_set_check(SphNet* n, uint32_t sides) {
+    if (!n || sides == 0 || (sides & ~(uint32_t)(SPH_CHECK_COMPRESSION | SPH_CHECK_RAREFACTION)))
+        return fail(SPH_ERR_INVALID_ARGUMENT, "sph_net_set_check: sides must be a non-empty set of SPH_CHECK_*");
+    n->checkSides = sides;
     return SPH_OK;
 }
 

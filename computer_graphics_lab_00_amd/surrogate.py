@@ -159,6 +159,11 @@ class SurrogateNet:
         """XSPH velocity smoothing at the start of each rollout macro-step (0: off)."""
         check(self._lib.sph_net_set_smoothing(self._h, smoothing))
 
+    def set_check(self, sides: int) -> None:
+        """Sides of the rollout's density check: capi.CHECK_COMPRESSION, CHECK_RAREFACTION
+        or both (the default)."""
+        check(self._lib.sph_net_set_check(self._h, sides))
+
     def set_backoff(self, max_backoff: int) -> None:
         """After a failed check, run 1, 2, 4, ... (at most max_backoff) macro-steps plain."""
         check(self._lib.sph_net_set_backoff(self._h, max_backoff))
@@ -424,6 +429,30 @@ def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, reco
 
 # ---- evaluation: plain path against the surrogate on one scene ----
 
+def densities(probe: FluidSolver, params: SphParams, pos4: np.ndarray, vel4: np.ndarray, dt_sub: float) -> np.ndarray:
+    """Per-particle densities of a state as the plain density pass computes them (the density
+    a substep from that state starts with), in particle order."""
+    probe.load_state(pos4, vel4)
+    probe.substep(params, dt_sub)
+    return probe.particles()[:, 3].copy()
+
+
+def step_error(solver: FluidSolver, net: "SurrogateNet", probe: FluidSolver, params: SphParams, dt: float,
+               dt_sub: float, threshold: float, stats: SphRolloutStats) -> dict:
+    """One rollout step of `solver` together with the plain step from the identical starting
+    state: per-particle |rho_surrogate - rho_plain| / rho0 of the two resulting states."""
+    pos0, vel0 = solver.state()
+    rollout(solver, net, params, dt, 1, threshold, stats)
+    ps, vs = solver.state()
+    probe.load_state(pos0, vel0)
+    probe.step(params, dt)
+    pp, vp = probe.state()
+    rs = densities(probe, params, ps, vs, dt_sub)
+    rp = densities(probe, params, pp, vp, dt_sub)
+    e = np.abs(rs.astype(np.float64) - rp.astype(np.float64)) / float(params.restDensity)
+    return {"mean": float(e.mean()), "p99": float(np.percentile(e, 99)), "max": float(e.max())}
+
+
 def _energy(pos, vel, params: SphParams) -> float:
     """Sum of 1/2 m |v|^2 - m g.x in FP64 (the conservation sum of the plain path)."""
     m = float(params.particleMass)
@@ -435,7 +464,7 @@ def _energy(pos, vel, params: SphParams) -> float:
 
 def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 / 600.0,
             threshold: float = DEFAULT_THRESHOLD, stats_every: int = 1, unchecked: bool = True, log=None,
-            progress_every: int = 20) -> dict:
+            progress_every: int = 20, error_every: int = 5) -> dict:
     """Runs `steps` FluidSolver.step(dt) calls of `workload` with the plain path and with
     the surrogate from the same initial state and returns timing and density/energy
     statistics. The timed runs carry no statistics work; the statistics come from
@@ -453,8 +482,11 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
         if log is not None:
             print(f"[surrogate] {msg}", file=log, flush=True)
 
-    def run(use_net: bool, timed: bool, label: str):
-        nonlocal threshold
+    probe = None
+    step_errors = []
+
+    def run(use_net: bool, timed: bool, label: str, errors: bool = False):
+        nonlocal threshold, probe
         solver.load_state(pos0, vel0)
         net.set_backoff(net.max_backoff)   # every run starts with no back-off pending
         st = SphRolloutStats()
@@ -462,7 +494,12 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
         solver.synchronize()
         t0 = time.perf_counter()
         for i in range(steps):
-            if use_net:
+            if use_net and errors and (i + 1) % error_every == 0:
+                # this step and the plain step from the same state, compared particle by particle
+                if probe is None:
+                    probe = FluidSolver(n, workload.cell_count)
+                step_errors.append({"step": i + 1, **step_error(solver, net, probe, p, dt, dt_sub, threshold, st)})
+            elif use_net:
                 rollout(solver, net, p, dt, 1, threshold, st)
             else:
                 solver.step(p, dt)
@@ -520,7 +557,7 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
               file=log, flush=True)
     _, _, plain_series = run(False, False, "plain statistics")
     e_plain = _energy(*solver.state(), p)
-    _, st2, sur_series = run(True, False, "surrogate statistics")
+    _, st2, sur_series = run(True, False, "surrogate statistics", errors=True)
     e_sur = _energy(*solver.state(), p)
     e0 = _energy(pos0, vel0, p)
     ps = np.array(plain_series)
@@ -535,6 +572,22 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     result["energy_drift_surrogate"] = (e_sur - e0) / abs(e0)
     result["series_plain"] = ps.tolist()
     result["series_surrogate"] = ss.tolist()
+    # mean deficit mean(max(1 - rho / rho0, 0)) = mean |dev| - mean excess
+    result["plain_density_deficit_mean"] = float((ps[:, 1] - ps[:, 0]).mean())
+    result["surrogate_density_deficit_mean"] = float((ss[:, 1] - ss[:, 0]).mean())
+    result["fallbacks_rarefaction"] = int(st.rarefactionFallbacks)
+    result["max_rarefaction_growth"] = float(st.maxRarefactionGrowth)
+    if step_errors:
+        result["step_error_vs_plain"] = {
+            "sampled_steps": len(step_errors), "every": error_every,
+            "mean": float(np.mean([e["mean"] for e in step_errors])),
+            "p99_mean": float(np.mean([e["p99"] for e in step_errors])),
+            "max_of_means": float(max(e["mean"] for e in step_errors)),
+            "max": float(max(e["max"] for e in step_errors)),
+            "series": step_errors,
+        }
+    if probe is not None:
+        probe.close()
     solver.close()
     return result
 

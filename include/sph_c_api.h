@@ -409,7 +409,18 @@ typedef struct SphRolloutStats {
     uint64_t backedOff;       /* macro-steps run plain while backing off after a failed check */
     float maxDeviation;       /* largest mean density excess seen by the check */
     float lastDeviation;
+    float maxRarefactionGrowth;   /* largest growth of the mean density deficit over a macro-step */
+    uint32_t pad0;
+    uint64_t rarefactionFallbacks;   /* fallbacks caused by the rarefaction side alone */
 } SphRolloutStats;
+
+/* The two sides of the rollout's density check. COMPRESSION fails a macro-step whose
+ * resulting mean density excess mean(max(rho / rho0 - 1, 0)) is above the threshold;
+ * RAREFACTION fails one that grew the mean density deficit mean(max(1 - rho / rho0, 0)) by
+ * more than the threshold (free surfaces carry a deficit of their own, so its growth over
+ * the macro-step is what is checked). */
+#define SPH_CHECK_COMPRESSION 1u
+#define SPH_CHECK_RAREFACTION 2u
 
 uint32_t sph_net_param_count(void);
 /* He-uniform weights from `seed` (splitmix64), zero biases. */
@@ -452,9 +463,10 @@ int sph_net_evaluate(SphNet* net, SphDataset* data, uint32_t begin, uint32_t end
 /* Feature rows of every particle of the solver's current state, in particle order. */
 int sph_features(SphSolver* solver, const SphParams* params, float dtSub, uint16_t* x);
 /* `steps` calls of FluidSolver::step(deltaSeconds) with the surrogate: every 4 substeps
- * are one network macro-step checked by the next density pass; a macro-step whose mean
- * max(rho / rho0 - 1, 0) exceeds `threshold` is redone by the plain path. Leftover
- * substeps, and every substep of scenes with colliders, run plain. */
+ * are one network macro-step checked by the next density pass; a macro-step that fails the
+ * check (see SPH_CHECK_*; mean excess above `threshold`, or mean deficit grown by more than
+ * `threshold`) is redone by the plain path. Leftover substeps, and every substep of scenes
+ * with colliders, run plain. */
 int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* params, float deltaSeconds, uint32_t steps,
                 float threshold, SphRolloutStats* stats);
 /* Optional XSPH smoothing (Monaghan 1989) in sph_rollout: at the start of a macro-step
@@ -462,6 +474,9 @@ int sph_rollout(SphSolver* solver, SphNet* net, const SphParams* params, float d
  * of its neighbourhood (sum s v_j / sum s), damping the particle-scale noise that
  * prediction errors inject. */
 int sph_net_set_smoothing(SphNet* net, float smoothing);
+/* The sides of the density check sph_rollout applies: SPH_CHECK_COMPRESSION,
+ * SPH_CHECK_RAREFACTION or both (the default). */
+int sph_net_set_check(SphNet* net, uint32_t sides);
 /* Back-off in sph_rollout: after a failed check the next 1, 2, 4, ... (at most
  * maxBackoff) macro-steps run plain before the network is tried again; a passed check
  * resets it. 0 (the default) tries the network on every macro-step. */
