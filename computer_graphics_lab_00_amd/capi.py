@@ -38,6 +38,7 @@ CFG_NO_GRAPH = 2
 CFG_BUCKET_WALK = 4
 CFG_TILED_DENSITY = 8
 CHECK_COMPRESSION, CHECK_RAREFACTION = 1, 2
+FEATURES_MOMENTS, FEATURES_KNN8 = 0, 1
 
 
 class SphCollider(C.Structure):
@@ -269,6 +270,10 @@ def _declare(lib):
         "sph_net_set_smoothing": (C.c_int, [P, C.c_float]),
         "sph_net_set_backoff": (C.c_int, [P, C.c_uint32]),
         "sph_net_set_check": (C.c_int, [P, C.c_uint32]),
+        "sph_net_set_features": (C.c_int, [P, C.c_uint32]),
+        "sph_net_features": (C.c_uint32, [P]),
+        "sph_dataset_set_features": (C.c_int, [P, C.c_uint32]),
+        "sph_knn_features": (C.c_int, [P, C.POINTER(SphParams), C.c_float, _HP]),
         "sph_density_deviation": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, _DP]),
         "sph_substep_device": (C.c_int, [C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

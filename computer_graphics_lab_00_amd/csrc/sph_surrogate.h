@@ -61,6 +61,22 @@ __device__ __forceinline__ void neighbourhood_features(float rho, float4 r0, flo
 
 hipError_t launch_features(const sphdev::Launch& L, const float4* pos, const float4* sums, float vScale, uint16_t* x,
                            hipStream_t s);
+
+// Feature kinds of a network's input rows.
+constexpr uint32_t kFeaturesMoments = 0;   // neighbourhood_features from the density pass's sums
+constexpr uint32_t kFeaturesKnn8 = 1;      // the 8 nearest neighbours (layout below)
+constexpr uint32_t kKnn = 8;
+
+// The 8-nearest-neighbour rows of the prepared state (density already in pos.w), slot order:
+// the neighbours are the in-range candidates of the density pass's walk (27 buckets, the
+// 32-slot cap, r2 < h2 computed as the pass does), distinct particles other than the
+// particle itself, the 8 smallest by (r2, particle index), nearest first. For neighbour k:
+//   6k .. 6k+2 : (x_i - x_j) / h
+//   6k+3 .. 6k+5 : (v_j - v_i) / vScale
+// (zeros for missing neighbours), then 48: 100 (rho / rho0 - 1), 49: h / kLengthScale.
+hipError_t launch_knn_features(const sphdev::Launch& L, const float4* pos, const float4* vel,
+                               const uint32_t* bucketStart, const uint32_t* counts, const uint32_t* ids,
+                               float vScale, uint16_t* x, hipStream_t s);
 hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, const float4* vel, const float* y,
                                   uint32_t ldY, float aScale, uint32_t substeps, const float4* sums,
                                   float smoothing, float4* posOut, float4* velOut, float4* posKeep,

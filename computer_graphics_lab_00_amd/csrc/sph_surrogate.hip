@@ -92,6 +92,7 @@ struct SphNet {
     uint64_t checkEpoch = 0;
     double checkDev[2] = {0.0, 0.0};
     uint32_t checkSides = 3;   // SPH_CHECK_COMPRESSION | SPH_CHECK_RAREFACTION
+    uint32_t featureKind = sphsur::kFeaturesMoments;   // what the input rows hold
     // The inference buffers (sums, ix, iy, keep*, the check's) are used on the stream of the
     // solver a call works with; a call on another solver's stream first waits for `done`,
     // recorded at the end of the previous call on `lastStream`.
@@ -122,6 +123,7 @@ struct SphNet {
 
 struct SphDataset {
     int device = 0;
+    uint32_t featureKind = sphsur::kFeaturesMoments;
     uint32_t cap = 0, size = 0;
     uint16_t* x = nullptr;
     float* t = nullptr;
@@ -408,8 +410,9 @@ extern "C" int sph_net_save(SphNet* n, const char* path) {
         if (!std::isfinite(v)) return fail(SPH_ERR_STATE, "refusing to save non-finite weights");
     FILE* f = fopen(path, "wb");
     if (!f) return fail(SPH_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
-    const uint32_t head[4] = {0x4E485053u /* "SPHN" */, 1u, shape_hash(), kParams};
-    bool ok = fwrite(head, 4, 4, f) == 4 && fwrite(p.data(), 4, kParams, f) == kParams;
+    // version 2: the feature kind follows the header (version 1 files are moment networks)
+    const uint32_t head[5] = {0x4E485053u /* "SPHN" */, 2u, shape_hash(), kParams, n->featureKind};
+    bool ok = fwrite(head, 4, 5, f) == 5 && fwrite(p.data(), 4, kParams, f) == kParams;
     ok = (fclose(f) == 0) && ok;
     return ok ? SPH_OK : fail(SPH_ERR_STATE, "write failed");
 }
@@ -418,14 +421,19 @@ extern "C" int sph_net_load(SphNet* n, const char* path) {
     if (!n || !path) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_net_load: null argument");
     FILE* f = fopen(path, "rb");
     if (!f) return fail(SPH_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
-    uint32_t head[4] = {0, 0, 0, 0};
+    uint32_t head[4] = {0, 0, 0, 0}, kind = sphsur::kFeaturesMoments;
     std::vector<float> p(kParams);
     bool ok = fread(head, 4, 4, f) == 4;
-    ok = ok && head[0] == 0x4E485053u && head[1] == 1u && head[2] == shape_hash() && head[3] == kParams;
+    ok = ok && head[0] == 0x4E485053u && (head[1] == 1u || head[1] == 2u) && head[2] == shape_hash() &&
+         head[3] == kParams;
+    if (ok && head[1] == 2u) ok = fread(&kind, 4, 1, f) == 1 && kind <= sphsur::kFeaturesKnn8;
     ok = ok && fread(p.data(), 4, kParams, f) == kParams;
     fclose(f);
     if (!ok) return fail(SPH_ERR_INVALID_ARGUMENT, "not a weight file of this network shape");
-    return sph_net_set_params(n, p.data());
+    if (int st = sph_net_set_params(n, p.data())) return st;
+    n->featureKind = kind;
+    n->predictedEpoch = 0;
+    return SPH_OK;
 }
 
 // Forward pass on host rows (x: rows x 64 BF16 bits) -> y: rows x 3.
@@ -539,7 +547,11 @@ extern "C" int sph_dataset_record(SphDataset* d, SphSolver* solver, const SphPar
     const sphdev::NeighbourSums ns{d->sums};
     if (int st = e.prepare(dtSub, &ns)) return st;
     const sphdev::Launch L = e.plan_launch();
-    SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), d->sums, sphsur::kVelocityScale, d->xAll, s));
+    if (d->featureKind == sphsur::kFeaturesKnn8)
+        SPH_HIP(sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_starts(), e.bucket_counts(),
+                                            e.sorted_ids(), sphsur::kVelocityScale, d->xAll, s));
+    else
+        SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), d->sums, sphsur::kVelocityScale, d->xAll, s));
     // `samples` distinct slots: a seeded partial Fisher-Yates shuffle.
     std::vector<uint32_t> perm(n);
     for (uint32_t i = 0; i < n; ++i) perm[i] = i;
@@ -709,9 +721,16 @@ int predict(SphNet* n, sphhost::Engine& e, uint32_t slot) {
     hipStream_t s = e.stream;
     const bool timed = (n->predictions++ & 7u) == 7u;   // not the first pass of a run: it warms up
     if (timed) SPH_HIP(hipEventRecord(n->ev[2 * slot], s));
-    SPH_HIP(sphmlp::launch_mlp_fused_sums(e.sorted_pos(), n->sums, L.total, e.sc[0].n, p.restDensity,
-                                          p.smoothingRadius, 1.0f / sphsur::kVelocityScale, n->wb, n->w,
-                                          reinterpret_cast<float4*>(n->iy), 0, s));
+    if (n->featureKind == sphsur::kFeaturesKnn8) {
+        // feature rows of the 8 nearest neighbours, then the network on the rows
+        SPH_HIP(sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_starts(), e.bucket_counts(),
+                                            e.sorted_ids(), sphsur::kVelocityScale, n->ix, s));
+        SPH_HIP(sphmlp::launch_mlp_fused(n->ix, e.sc[0].n, n->wb, n->w, reinterpret_cast<float4*>(n->iy), 0, s));
+    } else {
+        SPH_HIP(sphmlp::launch_mlp_fused_sums(e.sorted_pos(), n->sums, L.total, e.sc[0].n, p.restDensity,
+                                              p.smoothingRadius, 1.0f / sphsur::kVelocityScale, n->wb, n->w,
+                                              reinterpret_cast<float4*>(n->iy), 0, s));
+    }
     if (timed) SPH_HIP(hipEventRecord(n->ev[2 * slot + 1], s));
     n->timed[slot] = timed;
     n->predictedEpoch = e.epoch();
@@ -760,6 +779,51 @@ extern "C" int sph_features(SphSolver* solver, const SphParams* p, float dtSub, 
     hipFree(ordered);
     hipFree(sums);
     return st;
+}
+
+// The 8-nearest-neighbour rows (launch_knn_features) of every particle of the current state,
+// in particle order, rows x 64 BF16 bits.
+extern "C" int sph_knn_features(SphSolver* solver, const SphParams* p, float dtSub, uint16_t* x) {
+    sphhost::clear_error();
+    if (!solver || !p || !x) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_knn_features: null argument");
+    sphhost::Engine& e = solver->e;
+    if (int st = e.set_params(0, *p)) return st;
+    const uint32_t n = e.sc[0].n, N = e.total_slots();
+    uint16_t *all = nullptr, *ordered = nullptr;
+    SPH_HIP(hipMalloc(&all, (size_t)N * 64 * 2));
+    SPH_HIP(hipMalloc(&ordered, (size_t)N * 64 * 2));
+    int st = e.prepare(dtSub, nullptr);
+    if (!st) {
+        const sphdev::Launch L = e.plan_launch();
+        hipError_t he = sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_starts(),
+                                                    e.bucket_counts(), e.sorted_ids(), sphsur::kVelocityScale, all,
+                                                    e.stream);
+        if (he == hipSuccess) he = sphsur::launch_unpermute_rows(all, e.sorted_ids(), n, ordered, e.stream);
+        if (he == hipSuccess) he = hipMemcpyAsync(x, ordered, (size_t)n * 64 * 2, hipMemcpyDeviceToHost, e.stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
+        if (he != hipSuccess) st = hip_fail(he, "sph_knn_features");
+    }
+    hipFree(all);
+    hipFree(ordered);
+    return st;
+}
+
+// Which rows a network reads: SPH_FEATURES_MOMENTS (default) or SPH_FEATURES_KNN8. Weight
+// files keep it.
+extern "C" int sph_net_set_features(SphNet* n, uint32_t kind) {
+    if (!n || kind > sphsur::kFeaturesKnn8) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_net_set_features: bad kind");
+    n->featureKind = kind;
+    n->predictedEpoch = 0;
+    return SPH_OK;
+}
+
+extern "C" uint32_t sph_net_features(const SphNet* n) { return n ? n->featureKind : 0u; }
+
+// The rows sph_dataset_record writes from now on (SPH_FEATURES_*).
+extern "C" int sph_dataset_set_features(SphDataset* d, uint32_t kind) {
+    if (!d || kind > sphsur::kFeaturesKnn8) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_dataset_set_features: bad kind");
+    d->featureKind = kind;
+    return SPH_OK;
 }
 
 // FluidSolver::step with the surrogate: each run of 4 substeps becomes one network

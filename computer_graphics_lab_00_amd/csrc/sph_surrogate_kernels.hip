@@ -52,6 +52,105 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+__device__ inline uint32_t knn_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t mask) {
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & mask;
+}
+
+// (r2, id) before (r2', id'): nearer, or as near with the smaller particle index.
+__device__ __forceinline__ bool knn_before(float r, uint32_t id, float r2, uint32_t id2) {
+    return r < r2 || (r == r2 && id < id2);
+}
+
+// 8-nearest-neighbour feature rows (launch_knn_features in sph_surrogate.h), slot order.
+__global__ void __launch_bounds__(kBlock)
+    k_knn_features(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                   const uint32_t* __restrict__ bucketStart, const uint32_t* __restrict__ counts,
+                   const uint32_t* __restrict__ ids, float invV, uint16_t* __restrict__ x) {
+    const Where w = locate(L);
+    if (!w.valid) return;
+    const StepConsts k = w.d->k;
+    const uint32_t t = w.slot;
+    const uint32_t cellBase = w.d->cellOffset;
+    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+    const float4 p = pos[t];
+    const int32_t cx = (int32_t)floorf(p.x / k.h), cy = (int32_t)floorf(p.y / k.h), cz = (int32_t)floorf(p.z / k.h);
+    // the 8 nearest so far, sorted by (r2, id); slot for the duplicate test
+    float br[kKnn];
+    uint32_t bid[kKnn], bslot[kKnn];
+#pragma unroll
+    for (uint32_t q = 0; q < kKnn; ++q) {
+        br[q] = INFINITY;
+        bid[q] = 0xFFFFFFFFu;
+        bslot[q] = 0xFFFFFFFFu;
+    }
+    for (int32_t dz = -1; dz <= 1; ++dz)
+        for (int32_t dy = -1; dy <= 1; ++dy)
+            for (int32_t dx = -1; dx <= 1; ++dx) {
+                const uint32_t b = cellBase + knn_hash(cx + dx, cy + dy, cz + dz, k.cellMask);
+                uint32_t cnt = counts[b];
+                if (cnt == 0) continue;
+                if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+                const uint32_t lo = bucketStart[b];
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const uint32_t u = lo + j;
+                    if (u == t) continue;
+                    const float* q = reinterpret_cast<const float*>(pos + u);
+                    const float ddx = p.x - q[0], ddy = p.y - q[1], ddz = p.z - q[2];
+                    const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    if (!(r2 < k.h2)) continue;
+                    const uint32_t id = ids[u];
+                    if (!knn_before(r2, id, br[kKnn - 1], bid[kKnn - 1])) continue;
+                    bool dup = false;   // a bucket two neighbour cells share is visited twice
+#pragma unroll
+                    for (uint32_t e = 0; e < kKnn; ++e) dup = dup || bslot[e] == u;
+                    if (dup) continue;
+                    // insertion into the sorted registers, from the back
+#pragma unroll
+                    for (uint32_t e = kKnn - 1; e > 0; --e) {
+                        const bool here = knn_before(r2, id, br[e], bid[e]);
+                        const bool above = knn_before(r2, id, br[e - 1], bid[e - 1]);
+                        br[e] = here ? (above ? br[e - 1] : r2) : br[e];
+                        bid[e] = here ? (above ? bid[e - 1] : id) : bid[e];
+                        bslot[e] = here ? (above ? bslot[e - 1] : u) : bslot[e];
+                    }
+                    if (knn_before(r2, id, br[0], bid[0])) {
+                        br[0] = r2;
+                        bid[0] = id;
+                        bslot[0] = u;
+                    }
+                }
+            }
+    const float4 v = vel[t];
+    const float invH = 1.0f / k.h;
+    float f[kFeatures];
+#pragma unroll
+    for (int i = 0; i < kFeatures; ++i) f[i] = 0.0f;
+#pragma unroll
+    for (uint32_t e = 0; e < kKnn; ++e) {
+        if (bslot[e] == 0xFFFFFFFFu) continue;
+        const float4 q = pos[bslot[e]];
+        const float4 qv = vel[bslot[e]];
+        f[6 * e + 0] = (p.x - q.x) * invH;
+        f[6 * e + 1] = (p.y - q.y) * invH;
+        f[6 * e + 2] = (p.z - q.z) * invH;
+        f[6 * e + 3] = (qv.x - v.x) * invV;
+        f[6 * e + 4] = (qv.y - v.y) * invV;
+        f[6 * e + 5] = (qv.z - v.z) * invV;
+    }
+    f[6 * kKnn] = (p.w / k.restDensity - 1.0f) * kDensityScale;
+    f[6 * kKnn + 1] = k.h / kLengthScale;
+    uint4* row = reinterpret_cast<uint4*>(x + (size_t)t * kFeatures);
+#pragma unroll
+    for (int c = 0; c < kFeatures / 8; ++c) {
+        uint32_t wv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            wv[e] = (uint32_t)sphmlp::f32_to_bf16_bits(f[8 * c + 2 * e]) |
+                    ((uint32_t)sphmlp::f32_to_bf16_bits(f[8 * c + 2 * e + 1]) << 16);
+        row[c] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    }
+}
+
 __device__ void field_accel(const SphForceField& fd, const float p[3], float out[3]) {
     float off[3] = {p[0] - fd.position[0], p[1] - fd.position[1], p[2] - fd.position[2]};
     float distance = sqrtf((off[0] * off[0] + off[1] * off[1]) + off[2] * off[2]);
@@ -282,6 +381,13 @@ hipError_t launch_features(const Launch& L, const float4* pos, const float4* sum
                            hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
     k_features<<<L.blocks, kBlock, 0, s>>>(L, pos, sums, vScale, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_features(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
+                               const uint32_t* counts, const uint32_t* ids, float vScale, uint16_t* x, hipStream_t s) {
+    if (L.blocks == 0) return hipSuccess;
+    k_knn_features<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, ids, 1.0f / vScale, x);
     return hipGetLastError();
 }
 

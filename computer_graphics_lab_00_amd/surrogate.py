@@ -159,6 +159,15 @@ class SurrogateNet:
         """XSPH velocity smoothing at the start of each rollout macro-step (0: off)."""
         check(self._lib.sph_net_set_smoothing(self._h, smoothing))
 
+    @property
+    def feature_kind(self) -> int:
+        """capi.FEATURES_MOMENTS or capi.FEATURES_KNN8 (kept by save / load)."""
+        return int(self._lib.sph_net_features(self._h))
+
+    @feature_kind.setter
+    def feature_kind(self, kind: int) -> None:
+        check(self._lib.sph_net_set_features(self._h, kind))
+
     def set_check(self, sides: int) -> None:
         """Sides of the rollout's density check: capi.CHECK_COMPRESSION, CHECK_RAREFACTION
         or both (the default)."""
@@ -179,12 +188,13 @@ class SurrogateNet:
 class TrainingSet:
     """(feature row, target) pairs resident on the GPU."""
 
-    def __init__(self, capacity: int, device: int = 0):
+    def __init__(self, capacity: int, device: int = 0, feature_kind: int = capi.FEATURES_MOMENTS):
         self._lib = capi.load()
         h = C.c_void_p()
         check(self._lib.sph_dataset_create(device, capacity, C.byref(h)))
         self._h = h
         self.capacity = capacity
+        check(self._lib.sph_dataset_set_features(self._h, feature_kind))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -213,11 +223,12 @@ class TrainingSet:
         return x, t, t0
 
 
-def features(solver: FluidSolver, params: SphParams, dt_sub: float) -> np.ndarray:
+def features(solver: FluidSolver, params: SphParams, dt_sub: float, kind: int = capi.FEATURES_MOMENTS) -> np.ndarray:
     """Feature rows (BF16 bits) of every particle of the current state, particle order."""
     n = solver.particle_count()
     x = np.empty((n, FEATURES), np.uint16)
-    check(capi.load().sph_features(solver._h, C.byref(params), dt_sub, _hp(x)))
+    fn = capi.load().sph_knn_features if kind == capi.FEATURES_KNN8 else capi.load().sph_features
+    check(fn(solver._h, C.byref(params), dt_sub, _hp(x)))
     return x
 
 
@@ -364,6 +375,11 @@ def default_weights() -> Path:
     return Path(__file__).resolve().parent / "tuned" / "surrogate_mlp.bin"
 
 
+def knn_weights() -> Path:
+    """The network trained on the 8-nearest-neighbour rows."""
+    return Path(__file__).resolve().parent / "tuned" / "surrogate_knn8.bin"
+
+
 def record_workload(data: TrainingSet, workload: Workload, records: int, samples: int, warmup_steps: int = 0,
                     gap_steps: int = 2, dt: float = 1.0 / 600.0, seed: int = 99) -> tuple[int, int]:
     """Rows recorded from any workload's plain trajectory (e.g. the dam breaks the
@@ -390,6 +406,7 @@ def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, reco
                   samples: int = 4096, steps: int = 6000, batch: int = 8192, lr: float = 1e-3,
                   eval_workloads=(), dagger_rounds: int = 0, dam_every: int = 3, dam_warmup: int = 48,
                   log=sys.stderr) -> dict:
+    """(The rows are the network's feature kind.)"""
     """The training recipe: `scenes` trajectories, the last `held_out` kept for evaluation;
     Adam at lr for `steps` batches, then lr / 10 for a quarter as many. Each of
     `dagger_rounds` rounds then adds the same number of scenes recorded along the
@@ -397,7 +414,8 @@ def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, reco
     workload in `eval_workloads` contributes 8 records of 16384 rows from its own plain
     trajectory, used for evaluation only."""
     per_round = scenes * records * samples
-    data = TrainingSet(per_round * (1 + dagger_rounds) + len(eval_workloads) * 8 * 16384)
+    data = TrainingSet(per_round * (1 + dagger_rounds) + len(eval_workloads) * 8 * 16384,
+                       feature_kind=net.feature_kind)
     ranges = generate(data, scenes, records, samples, dam_every=dam_every, dam_warmup=dam_warmup, log=log)
     split = ranges[scenes - held_out][0] if held_out else len(data)
     end = len(data)
@@ -612,6 +630,8 @@ def main(argv=None) -> int:
     t.add_argument("--dam-warmup", type=int, default=48, help="dam columns are recorded after 0 .. this many - 1 steps")
     t.add_argument("--eval", nargs="*", default=["m1"], choices=("m1", "r4m", "s16k"),
                    help="dam breaks whose own trajectories the network is also evaluated on")
+    t.add_argument("--features", choices=("moments", "knn8"), default="moments",
+                   help="network input: neighbourhood moments or the 8 nearest neighbours")
     e = sp.add_parser("compare", help="plain path against the surrogate on a dam break")
     e.add_argument("--weights", default=str(default_weights()))
     e.add_argument("--workload", choices=("m1", "r4m", "s16k"), default="m1")
@@ -647,6 +667,7 @@ def main(argv=None) -> int:
     if a.cmd == "train":
         import time
         net = SurrogateNet(seed=a.seed)
+        net.feature_kind = capi.FEATURES_KNN8 if a.features == "knn8" else capi.FEATURES_MOMENTS
         t0 = time.perf_counter()
         from . import fluid
         wls = [{"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[w]() for w in a.eval]

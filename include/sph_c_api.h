@@ -462,6 +462,20 @@ int sph_net_evaluate(SphNet* net, SphDataset* data, uint32_t begin, uint32_t end
 
 /* Feature rows of every particle of the solver's current state, in particle order. */
 int sph_features(SphSolver* solver, const SphParams* params, float dtSub, uint16_t* x);
+
+/* The input a network reads. MOMENTS: 20 permutation-invariant sums over the neighbourhood
+ * the density pass accumulates (the default). KNN8: the relative positions (/ h) and
+ * velocities of the particle's 8 nearest in-range neighbours (distinct particles, by
+ * (r2, index), nearest first; zeros where fewer), its density deviation (x 100) and h / 1 cm
+ * — the 8-nearest-neighbour input of the surrogate-training configuration, computed by an
+ * extra pass over the same 27 buckets. Weight files keep the kind. */
+#define SPH_FEATURES_MOMENTS 0u
+#define SPH_FEATURES_KNN8 1u
+int sph_net_set_features(SphNet* net, uint32_t kind);
+uint32_t sph_net_features(const SphNet* net);
+int sph_dataset_set_features(SphDataset* data, uint32_t kind);
+/* KNN8 rows of every particle of the solver's current state, in particle order. */
+int sph_knn_features(SphSolver* solver, const SphParams* params, float dtSub, uint16_t* x);
 /* `steps` calls of FluidSolver::step(deltaSeconds) with the surrogate: every 4 substeps
  * are one network macro-step checked by the next density pass; a macro-step that fails the
  * check (see SPH_CHECK_*; mean excess above `threshold`, or mean deficit grown by more than

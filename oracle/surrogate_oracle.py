@@ -248,3 +248,47 @@ def targets(forces_and_density):
         a = np.asarray(f, dtype=F32)[:, :3] / rho[:, None]
         acc = a if acc is None else acc + a
     return acc * (F32(1.0) / (F32(len(forces_and_density)) * F32(ACCEL_SCALE)))
+
+
+def knn_features(pos4, vel4, density, h, cell_count, rest_density, cap=32, v_scale=VELOCITY_SCALE, k=8):
+    """The 8-nearest-neighbour rows (n x 64, FP32 values before BF16 rounding) in particle
+    order: among the in-range candidates of the density walk (27 buckets in the reference
+    order, the first `cap` members of each bucket by index, r2 < h2 in the pass's FP32
+    order), the distinct particles j != i with the k smallest (r2, j); for each, (x_i - x_j)
+    * (1 / h) and (v_j - v_i) * (1 / v_scale); then 100 (rho / rho0 - 1) and h / 1 cm."""
+    p = np.asarray(pos4, dtype=F32)[:, :3]
+    v = np.asarray(vel4, dtype=F32)[:, :3]
+    rho = np.asarray(density, dtype=F32)
+    n = p.shape[0]
+    hf = F32(h)
+    cells = np.floor(p / hf).astype(np.int64)
+    buckets = _hash(cells, cell_count)
+    order = np.lexsort((np.arange(n), buckets))
+    members = {}
+    for i in order:
+        lst = members.setdefault(int(buckets[i]), [])
+        if len(lst) < cap:
+            lst.append(int(i))
+    offsets = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    h2 = hf * hf
+    inv_h = F32(1) / hf
+    inv_v = F32(1) / F32(v_scale)
+    f = np.zeros((n, FEATURES), F32)
+    for i in range(n):
+        seen = {}
+        for off in offsets:
+            b = int(_hash((cells[i] + np.array(off))[None, :], cell_count)[0])
+            for j in members.get(b, ()):
+                if j == i or j in seen:
+                    continue
+                d3 = p[i] - p[j]
+                r2 = (d3[0] * d3[0] + d3[1] * d3[1]) + d3[2] * d3[2]
+                if r2 < h2:
+                    seen[j] = r2
+        near = sorted(seen.items(), key=lambda kv: (kv[1], kv[0]))[:k]
+        for e, (j, _) in enumerate(near):
+            f[i, 6 * e: 6 * e + 3] = (p[i] - p[j]) * inv_h
+            f[i, 6 * e + 3: 6 * e + 6] = (v[j] - v[i]) * inv_v
+    f[:, 6 * k] = (rho / F32(rest_density) - F32(1.0)) * F32(DENSITY_SCALE)
+    f[:, 6 * k + 1] = hf / F32(LENGTH_SCALE)
+    return f

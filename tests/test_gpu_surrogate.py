@@ -201,6 +201,22 @@ def test_features_match_the_oracle_bit_for_bit(sph, oracle_lib):
     assert np.array_equal(pos2[:, :3], pos[:, :3]) and np.array_equal(vel2[:, :3], vel[:, :3])
 
 
+def test_knn_features_match_the_oracle_bit_for_bit(sph, oracle_lib):
+    """The 8-nearest-neighbour rows against their numpy restatement: the same candidates as
+    the density walk, distinct neighbours by (r2, index), BF16 bits equal."""
+    p, pos, vel, s = _small_scene()
+    dt_sub = (1.0 / 600.0) / fluid.fluid_substeps(p, 1.0 / 600.0)
+    say(f"knn features n={pos.shape[0]}")
+    x = surrogate.features(s, p, dt_sub, capi.FEATURES_KNN8)
+    rho, _ = _oracle_density(oracle_lib, p, pos, vel, dt_sub)
+    want = SO.bf16_bits(SO.knn_features(pos, vel, rho, p.smoothingRadius, p.cellCount, p.restDensity))
+    full = (SO.from_bits(x)[:, 42:48] != 0).any(axis=1).mean()
+    say(f"  particles with 8 neighbours: {full:.3f}")
+    assert full > 0.9
+    diff = x != want
+    assert not diff.any(), f"{diff.sum()} values differ, first at {np.argwhere(diff)[:3].tolist()}"
+
+
 def test_training_targets_match_the_oracle(sph, oracle_lib):
     p, pos, vel, s = _small_scene(seed=4)
     n = pos.shape[0]
@@ -501,9 +517,10 @@ def _xsph(p, x, v, smoothing, c_poly6):
     return out
 
 
-@pytest.mark.parametrize("lattice,smoothing", [((16, 16, 8), 0.0), ((16, 16, 8), surrogate.DEFAULT_SMOOTHING),
-                                               ((20, 10, 10), surrogate.DEFAULT_SMOOTHING)])
-def test_rollout_equals_features_forward_and_restated_macro_steps(sph, lattice, smoothing):
+@pytest.mark.parametrize("lattice,smoothing,kind", [((16, 16, 8), 0.0, 0), ((16, 16, 8), surrogate.DEFAULT_SMOOTHING, 0),
+                                                    ((20, 10, 10), surrogate.DEFAULT_SMOOTHING, 0),
+                                                    ((16, 16, 8), surrogate.DEFAULT_SMOOTHING, 1)])
+def test_rollout_equals_features_forward_and_restated_macro_steps(sph, lattice, smoothing, kind):
     """The rollout builds its feature rows inside the fused network kernel; restated
     with the separate kernels instead (sph_features -> sph_net_forward), the XSPH step from
     the oracle's neighbourhood sums and the macro-step in numpy, two macro-steps of the
@@ -511,9 +528,14 @@ def test_rollout_equals_features_forward_and_restated_macro_steps(sph, lattice, 
     particles (not a multiple of the 256-lane workgroup: the last workgroup's idle lanes)."""
     p, pos, vel, s = _small_scene(seed=10, lattice=lattice)
     n = pos.shape[0]
-    say(f"rollout == features + forward + restated macro-steps, n={n}, smoothing={smoothing}")
+    say(f"rollout == features + forward + restated macro-steps, n={n}, smoothing={smoothing}, kind={kind}")
     net = surrogate.SurrogateNet(seed=3)
-    net.load(surrogate.default_weights())
+    if kind == capi.FEATURES_KNN8 and surrogate.knn_weights().exists():
+        net.load(surrogate.knn_weights())
+    else:
+        net.load(surrogate.default_weights())
+        net.feature_kind = kind
+    assert net.feature_kind == kind
     net.set_smoothing(smoothing)
     dt_sub = F32(F32(1.0 / 600.0) / F32(8))
     consts = (C.c_float * 8)()
@@ -523,7 +545,7 @@ def test_rollout_equals_features_forward_and_restated_macro_steps(sph, lattice, 
     probe = fluid.FluidSolver(n, 1 << 14)
     for _ in range(2):
         probe.load_state(x, v)
-        y = net.forward(surrogate.features(probe, p, float(dt_sub)))[:, :3]
+        y = net.forward(surrogate.features(probe, p, float(dt_sub), kind))[:, :3]
         assert np.abs(y).max() > 0.01
         if smoothing > 0:
             v = _xsph(p, x, v, smoothing, consts[2])
