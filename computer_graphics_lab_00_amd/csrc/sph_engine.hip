@@ -324,6 +324,9 @@ void Engine::touch() { epoch_ = ++g_epoch; }
 
 void Engine::invalidate() {
     planKey.clear();
+    // A replay of either graph may still be running on the stream: wait for it before the
+    // executable graphs are destroyed.
+    if ((graph[0] || graph[1]) && stream) hipStreamSynchronize(stream);
     for (int i = 0; i < 2; ++i) {
         if (graph[i]) hipGraphExecDestroy(graph[i]);
         graph[i] = nullptr;
