@@ -254,9 +254,14 @@ def surrogate_leg(w, steps: int) -> dict | None:
         "roofline": None,
     }
     if r.get("mlp_tflops"):
+        useful = r["mlp_tflops"] * S.USEFUL_MLP_FLOP_PER_ROW / S.ROLLOUT_MLP_FLOP_PER_ROW
         out["roofline"] = {"bound": "mfma", "kernel": "mlp_fused", "achieved": r["mlp_tflops"], "peak": 2500.0,
                            "unit": "TFLOP/s", "frac": r["mlp_tflops"] / 2500.0,
-                           "flop_per_unit": S.ROLLOUT_MLP_FLOP_PER_ROW, "ms_per_macro_step": r["mlp_ms_per_macro_step"]}
+                           "flop_per_unit": S.ROLLOUT_MLP_FLOP_PER_ROW, "ms_per_macro_step": r["mlp_ms_per_macro_step"],
+                           "note": "executed MFMA FLOP (inputs padded to 32, outputs to 32)",
+                           "useful": {"achieved": useful, "frac": useful / 2500.0,
+                                      "flop_per_unit": S.USEFUL_MLP_FLOP_PER_ROW,
+                                      "note": "20 live inputs, 3 live outputs"}}
     return out
 
 

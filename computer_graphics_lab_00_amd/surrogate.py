@@ -36,6 +36,9 @@ OUTPUTS = 3
 # zero inputs 20..63: 2 (32 x 128 + 128 x 128 + 128 x 128 + 128 x 32) per row.
 MLP_FLOP_PER_ROW = 2 * (64 * 128 + 128 * 128 + 128 * 128 + 128 * 32)
 ROLLOUT_MLP_FLOP_PER_ROW = 2 * (32 * 128 + 128 * 128 + 128 * 128 + 128 * 32)
+# The useful part: 20 live inputs and 3 live outputs of the moment network
+# (2 (20 x 128 + 128 x 128 + 128 x 128 + 128 x 3) per row); the rest is padding to MFMA tiles.
+USEFUL_MLP_FLOP_PER_ROW = 2 * (20 * 128 + 128 * 128 + 128 * 128 + 128 * 3)
 MACRO = 4
 DEFAULT_THRESHOLD = 0.01
 # Rollout policy of the measured configuration, chosen on unseen dam columns of the training
