@@ -27,7 +27,10 @@ STATUS_NAMES = {
     5: "SPH_ERR_CAPACITY",
     6: "SPH_ERR_NOT_POWER_OF_TWO",
     7: "SPH_ERR_STATE",
+    8: "SPH_ERR_DEGENERATE",
 }
+ERR_DEGENERATE = 8
+MAX_BIN_OCCUPANCY = 16384
 
 SHAPE_SPHERE, SHAPE_BOX, SHAPE_PLANE, SHAPE_CYLINDER, SHAPE_CAPSULE, SHAPE_MESH = range(6)
 FIELD_WIND, FIELD_VORTEX, FIELD_POINT = range(3)
@@ -206,6 +209,7 @@ def _declare(lib):
         "sph_step": (C.c_int, [P, C.POINTER(SphParams), C.c_float]),
         "sph_substep": (C.c_int, [P, C.POINTER(SphParams), C.c_float]),
         "sph_synchronize": (C.c_int, [P]),
+        "sph_max_bin_occupancy": (C.c_int, [P, C.POINTER(C.c_uint32)]),
         "sph_particle_count": (C.c_uint32, [P]),
         "sph_read_particles": (C.c_int, [P, _FP, _FP]),
         "sph_read_previous": (C.c_int, [P, _FP]),

@@ -91,6 +91,14 @@ extern "C" int sph_synchronize(SphSolver* s) {
     return s->e.synchronize();
 }
 
+extern "C" int sph_max_bin_occupancy(SphSolver* s, uint32_t* out) {
+    sphhost::clear_error();
+    if (!s || !out) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_max_bin_occupancy: null argument");
+    if (int st = s->e.synchronize()) return st;
+    *out = s->e.last_bin_max();
+    return SPH_OK;
+}
+
 extern "C" uint32_t sph_particle_count(const SphSolver* s) { return s ? s->e.sc[0].n : 0u; }
 
 extern "C" int sph_read_particles(SphSolver* s, float* pos4, float* vel4) {

@@ -131,6 +131,10 @@ struct GridBuffers {
     // For the tiled density pass (set when the launch has half-tiles): the list of particles
     // whose own cell lies outside the half-tile they are sorted into.
     uint2* rest = nullptr;           // (slot, scene)
+    // When set: the largest bin occupancy of the build (zeroed with the counts, raised by
+    // k_bin_slots), and its copy in mapped host memory, written by k_rank_reorder.
+    uint32_t* binMax = nullptr;
+    uint32_t* binMaxHost = nullptr;
 };
 
 // Logical block for a hardware workgroup id. Workgroups are dispatched round-robin over

@@ -104,6 +104,8 @@ public:
     uint32_t total_slots() const { return totalN; }
     int ensure_force_buffer();
     uint32_t planned_blocks() const { return planBlocks; }
+    // Largest bin occupancy of the last grid build the device has finished (0 before any).
+    uint32_t last_bin_max() const { return binMaxHost ? __atomic_load_n(binMaxHost, __ATOMIC_RELAXED) : 0u; }
 
     std::vector<SceneHost> sc;
     std::vector<uint32_t> schedule;   // empty: scene order
@@ -117,6 +119,8 @@ private:
     int run_substep(uint32_t j, bool prepareOnly = false, const sphdev::NeighbourSums* sums = nullptr,
                     bool gridOnly = false);
     int enqueue_step(uint32_t maxSub);
+    int check_occupancy();
+    void reset_occupancy();
     void invalidate();
     sphdev::Launch launch_for(uint32_t j) const;
 
@@ -142,6 +146,9 @@ private:
     uint32_t maxTiles = 0, planTiles = 0;
     uint2* restList = nullptr;        // particles the tiled density pass leaves to its global walk
     uint32_t* restCount = nullptr;
+    uint32_t* binMax = nullptr;         // device word: largest bin occupancy of the current build
+    uint32_t* binMaxHost = nullptr;     // its copy in mapped host memory (host address)
+    uint32_t* binMaxHostDev = nullptr;  // the same memory's device address
     void* hostStage = nullptr;
     size_t hostStageBytes = 0;
     std::vector<uint8_t> planKey;

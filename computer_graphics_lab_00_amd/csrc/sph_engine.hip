@@ -210,6 +210,10 @@ void Engine::free_all() {
     hipFree(dTileScene);
     hipFree(restList);
     hipFree(restCount);
+    hipFree(binMax);
+    binMax = nullptr;
+    if (binMaxHost) hipHostFree(binMaxHost);
+    binMaxHost = binMaxHostDev = nullptr;
     if (hostStage) hipHostFree(hostStage);
     hostStage = nullptr;
     for (auto& t : timed) {
@@ -307,6 +311,12 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     hostStageBytes = kMaxSubsteps * S * sizeof(sphdev::SceneDesc) + S * sizeof(sphdev::BoundaryConsts) +
                      (maxBlocks + 1) * sizeof(uint16_t) + ((size_t)maxTiles + 1) * sizeof(uint16_t) + 512;
     if (e == hipSuccess) e = hipHostMalloc(&hostStage, hostStageBytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&binMax, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc(&binMaxHost, sizeof(uint32_t), hipHostMallocMapped);
+    if (e == hipSuccess) {
+        *binMaxHost = 0u;
+        e = hipHostGetDevicePointer((void**)&binMaxHostDev, binMaxHost, 0);
+    }
     if (e != hipSuccess) {
         int st = e == hipErrorOutOfMemory ? SPH_ERR_OUT_OF_MEMORY : SPH_ERR_DEVICE;
         std::string msg = std::string("engine create: ") + hipGetErrorString(e);
@@ -368,6 +378,7 @@ int Engine::emit(uint32_t scene, const SphParams& p, uint32_t count) {
     s.n = count;
     hasGrid = false;
     touch();
+    reset_occupancy();
     SPH_HIP(sphdev::launch_emit(p, count, pos[pc] + s.offset, vel[pc] + s.offset, prev + s.offset,
                                 ids[ic] + s.offset, stream));
     return SPH_OK;
@@ -382,6 +393,7 @@ int Engine::load_state(uint32_t scene, const float* pos4, const float* vel4, uin
     s.n = count;
     hasGrid = false;
     touch();
+    reset_occupancy();
     if (count == 0) return SPH_OK;
     const size_t bytes = (size_t)count * sizeof(float4);
     SPH_HIP(hipMemcpyAsync(pos[pc] + s.offset, pos4, bytes, hipMemcpyHostToDevice, stream));
@@ -570,6 +582,8 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
     g.bucketStart = bucketStart;
     g.restCount = restCount;
     g.rest = restList;
+    g.binMax = binMax;
+    g.binMaxHost = binMaxHostDev;
     {
         Timer t(this, KC_FILL);
         SPH_HIP(sphdev::launch_grid_clear(L, g, stream));
@@ -621,8 +635,29 @@ int Engine::enqueue_step(uint32_t maxSub) {
     return SPH_OK;
 }
 
+// A new state: the occupancy of grids built from the old one no longer applies. Grid builds
+// still in flight write their copy when they finish, so they are waited for first.
+void Engine::reset_occupancy() {
+    if (!binMaxHost) return;
+    hipStreamSynchronize(stream);
+    __atomic_store_n(binMaxHost, 0u, __ATOMIC_RELAXED);
+}
+
+// The grid build ranks each particle against every member of its spatial bin, so its cost
+// grows with the square of the largest bin occupancy. A state that piled SPH_MAX_BIN_OCCUPANCY
+// or more particles into one bin (an unstable run collapsing onto a wall or a corner) is
+// refused rather than stepped at that cost. The figure is the last finished build's.
+int Engine::check_occupancy() {
+    const uint32_t m = last_bin_max();
+    if (m < SPH_MAX_BIN_OCCUPANCY) return SPH_OK;
+    return fail(SPH_ERR_DEGENERATE, "a grid bin holds " + std::to_string(m) +
+                                        " particles (SPH_MAX_BIN_OCCUPANCY is " +
+                                        std::to_string(SPH_MAX_BIN_OCCUPANCY) + "): the state has piled up");
+}
+
 int Engine::step(float dt, bool singleSubstep) {
     if (int st = set_device()) return st;
+    if (int st = check_occupancy()) return st;
     touch();
     uint32_t maxSub = 0;
     if (int st = ensure_plan(dt, singleSubstep, &maxSub)) return st;
@@ -656,6 +691,7 @@ int Engine::step(float dt, bool singleSubstep) {
 
 int Engine::prepare(float dt, const sphdev::NeighbourSums* sums) {
     if (int st = set_device()) return st;
+    if (int st = check_occupancy()) return st;
     touch();
     uint32_t maxSub = 0;
     if (int st = ensure_plan(dt, true, &maxSub)) return st;
@@ -665,6 +701,7 @@ int Engine::prepare(float dt, const sphdev::NeighbourSums* sums) {
 
 int Engine::build_grid() {
     if (int st = set_device()) return st;
+    if (int st = check_occupancy()) return st;
     touch();
     // The grid does not depend on dt: re-validate the last plan (its tables are rebuilt if
     // counts or params changed since), or make one for a 1/600 s step if there is none.

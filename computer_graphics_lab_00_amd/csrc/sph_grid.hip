@@ -77,9 +77,11 @@ __device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ul
 // counts = 0 and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
 __global__ void __launch_bounds__(kBlock)
     k_grid_clear(uint32_t* __restrict__ counts, uint32_t* __restrict__ rep, uint32_t cells,
-                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount) {
+                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount,
+                 uint32_t* __restrict__ binMax) {
     const uint32_t stride = gridDim.x * kBlock;
     if (restCount && blockIdx.x == 0 && threadIdx.x == 0) *restCount = 0u;
+    if (binMax && blockIdx.x == 0 && threadIdx.x == 0) *binMax = 0u;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride) {
         counts[i] = 0u;
         rep[i] = 0xFFFFFFFFu;
@@ -140,7 +142,8 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void __launch_bounds__(kBlock)
     k_bin_slots(Launch L, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rep,
-                uint32_t* __restrict__ binCount, uint32_t* __restrict__ binOf, uint32_t* __restrict__ slots) {
+                uint32_t* __restrict__ binCount, uint32_t* __restrict__ binOf, uint32_t* __restrict__ slots,
+                uint32_t* __restrict__ binMax) {
     const Where w = locate(L);
     const uint32_t lane = lane_id();
     const uint32_t bin = w.valid ? rep[keys[w.slot]] : 0xFFFFFFFFu;
@@ -158,9 +161,23 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t mine = heads & lanes_upto(lane);
     const uint32_t headLane = mine ? 63u - (uint32_t)__clzll((unsigned long long)mine) : lane;
     base = __shfl(base, headLane, 64);
+    const uint32_t slot = base + (lane - headLane);
     if (w.valid) {
         binOf[w.slot] = bin;
-        slots[w.slot] = base + (lane - headLane);
+        slots[w.slot] = slot;
+    }
+    if (binMax) {
+        // The largest bin's last member gets slot (occupancy - 1): the maximum of slot + 1
+        // over all particles is the largest bin occupancy. One atomic per wave at most, and
+        // none once the running maximum is at least this wave's.
+        uint32_t m = w.valid ? slot + 1u : 0u;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const uint32_t o = __shfl_xor(m, d, 64);
+            m = o > m ? o : m;
+        }
+        if (lane == 0u && m > __hip_atomic_load(binMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(binMax, m);
     }
 }
 
@@ -264,7 +281,12 @@ __global__ void __launch_bounds__(kBlock)
                    const float4* __restrict__ pos, const float4* __restrict__ vel, float4* __restrict__ posSorted,
                    float4* __restrict__ velSorted, uint32_t* __restrict__ idsSorted,
                    uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart, uint2* __restrict__ rest,
-                   uint32_t* __restrict__ restCount) {
+                   uint32_t* __restrict__ restCount, const uint32_t* __restrict__ binMax,
+                   uint32_t* __restrict__ binMaxHost) {
+    // The largest bin occupancy of this build (final: k_bin_slots has completed) to the
+    // host's mapped copy, by one lane.
+    if (binMaxHost && blockIdx.x == 0 && threadIdx.x == 0)
+        *binMaxHost = __hip_atomic_load(binMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const Where w = locate(L);
     const uint2 me = w.valid ? pairKI[w.slot] : make_uint2(0u, 0u);
     const uint32_t src = w.valid ? pairSrc[w.slot] : 0u;
@@ -385,7 +407,8 @@ hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t 
     const uint32_t span = L.totalCells > L.totalBins ? L.totalCells : L.totalBins;
     uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
-    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount);
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount,
+                                           b.binMax);
     return hipGetLastError();
 }
 
@@ -399,7 +422,7 @@ hipError_t launch_keys(const Launch& L, const GridBuffers& b, hipStream_t s) {
 }
 
 hipError_t launch_bin_slots(const Launch& L, const GridBuffers& b, hipStream_t s) {
-    k_bin_slots<<<L.blocks, kBlock, 0, s>>>(L, b.keys, b.rep, b.binStart, b.binOf, b.slots);
+    k_bin_slots<<<L.blocks, kBlock, 0, s>>>(L, b.keys, b.rep, b.binStart, b.binOf, b.slots, b.binMax);
     return hipGetLastError();
 }
 
@@ -419,7 +442,8 @@ hipError_t launch_rank_reorder(const Launch& L, const GridBuffers& b, hipStream_
     const bool tiles = L.tiles > 0 && b.rest && b.restCount;
     k_rank_reorder<<<L.blocks, kBlock, 0, s>>>(L, b.pairKI, b.pairSrc, b.binOf, b.binStart, b.pos, b.vel,
                                               b.posSorted, b.velSorted, b.idsSorted, b.keysSorted, b.bucketStart,
-                                              tiles ? b.rest : nullptr, b.restCount);
+                                              tiles ? b.rest : nullptr, b.restCount, b.binMax,
+                                              b.binMax ? b.binMaxHost : nullptr);
     return hipGetLastError();
 }
 

@@ -134,6 +134,13 @@ class FluidSolver:
     def synchronize(self) -> None:
         check(self._lib.sph_synchronize(self._h))
 
+    def max_bin_occupancy(self) -> int:
+        """Particles in the fullest spatial bin of the last grid build (synchronous); steps
+        are refused with SPH_ERR_DEGENERATE once it reaches capi.MAX_BIN_OCCUPANCY."""
+        out = C.c_uint32()
+        check(self._lib.sph_max_bin_occupancy(self._h, C.byref(out)))
+        return int(out.value)
+
     def velocities(self) -> np.ndarray:
         out = np.empty((self.particle_count(), 4), np.float32)
         check(self._lib.sph_read_particles(self._h, None, _fptr(out)))

@@ -33,6 +33,12 @@ extern "C" {
 #define SPH_CELL_CAPACITY 32u
 /* src/physics/fluid_sph.h:76 (FLUID_MAX_FRAME_STEP). */
 #define SPH_MAX_FRAME_STEP (1.0f / 30.0f)
+/* No counterpart in the reference, whose grid stores at most FLUID_CELL_CAPACITY particles
+ * per cell (src/physics/fluid_sph.cpp:160-188). This grid keeps every particle and orders
+ * each spatial bin's members by rank, at a cost quadratic in the bin's occupancy; a step
+ * from a state whose last grid held this many particles in one bin returns
+ * SPH_ERR_DEGENERATE. */
+#define SPH_MAX_BIN_OCCUPANCY 16384u
 
 /* Status codes. */
 typedef enum SphStatus {
@@ -43,7 +49,8 @@ typedef enum SphStatus {
     SPH_ERR_DEVICE = 4,          /* a HIP runtime call failed */
     SPH_ERR_CAPACITY = 5,        /* count or cellCount above what the handle was created for */
     SPH_ERR_NOT_POWER_OF_TWO = 6,
-    SPH_ERR_STATE = 7            /* call not valid in the current state (e.g. read before emit) */
+    SPH_ERR_STATE = 7,           /* call not valid in the current state (e.g. read before emit) */
+    SPH_ERR_DEGENERATE = 8       /* the state has piled SPH_MAX_BIN_OCCUPANCY particles into one bin */
 } SphStatus;
 
 /* scene::ColliderShape (src/scene/scene.h:119-131). */
@@ -219,6 +226,12 @@ int sph_substep(SphSolver* solver, const SphParams* params, float dt);
 
 /* Block until the solver's stream is idle. */
 int sph_synchronize(SphSolver* solver);
+
+/* No counterpart in the reference: the largest number of particles in one spatial bin of
+ * the last grid build (0 before the first, and after sph_emit / sph_load_state).
+ * Synchronous. sph_step, sph_substep and the grid builds return SPH_ERR_DEGENERATE while
+ * it is SPH_MAX_BIN_OCCUPANCY or more; sph_load_state or sph_emit starts over. */
+int sph_max_bin_occupancy(SphSolver* solver, uint32_t* out);
 
 /* FluidSolver::particleCount (src/physics/fluid_sph.h:108). */
 uint32_t sph_particle_count(const SphSolver* solver);

@@ -536,6 +536,44 @@ def test_api_errors_and_noops():
             gpu.forces()                                    # created without keep_forces
 
 
+def test_bin_occupancy_guard():
+    """cellCount = 1 puts every particle into bucket 0, and so into one spatial bin: the
+    fullest bin holds exactly n. At n = SPH_MAX_BIN_OCCUPANCY the next step is refused
+    with SPH_ERR_DEGENERATE; loading a state starts over."""
+    limit = capi.MAX_BIN_OCCUPANCY
+    w = fluid.s16k()
+    assert w.count == limit
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    dt = 1.0 / 600.0 / fluid.fluid_substeps(p, 1.0 / 600.0)
+    with fluid.FluidSolver(w.count, 1 << 16) as gpu:
+        say("case: bin occupancy, 1000 particles in one bucket")
+        q = w.params()
+        q.cellCount = 1
+        gpu.load_state(pos[:1000], vel[:1000])
+        assert gpu.max_bin_occupancy() == 0
+        gpu.substep(q, dt)
+        assert gpu.max_bin_occupancy() == 1000
+        gpu.step(q, 1.0 / 600.0)                            # below the limit: runs
+        say("case: bin occupancy, the whole s16k lattice in one bucket")
+        gpu.load_state(pos, vel)
+        gpu.substep(q, dt)
+        assert gpu.max_bin_occupancy() == limit
+        with pytest.raises(capi.SphError) as e:
+            gpu.step(q, 1.0 / 600.0)
+        assert e.value.status == capi.ERR_DEGENERATE
+        with pytest.raises(capi.SphError) as e:
+            gpu.substep(p, dt)
+        assert e.value.status == capi.ERR_DEGENERATE
+        say("case: bin occupancy, the lattice hashed normally")
+        gpu.load_state(pos, vel)
+        assert gpu.max_bin_occupancy() == 0
+        gpu.step(p, 1.0 / 600.0)
+        m = gpu.max_bin_occupancy()
+        _, counts, _ = gpu.grid(p.cellCount)
+        assert int(counts.max()) <= m < limit   # a bin holds whole buckets
+
+
 def test_keep_rendered_and_previous():
     say("case: keep_rendered")
     p = fluid.derive_fluid_params(SphSettings(), fluid.translation4(0.0, 1.0, 0.0), 512, 1024)
