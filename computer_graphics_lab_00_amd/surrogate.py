@@ -19,6 +19,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import sys
+import time
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -39,6 +40,10 @@ ROLLOUT_MLP_FLOP_PER_ROW = 2 * (32 * 128 + 128 * 128 + 128 * 128 + 128 * 32)
 # The useful part: 20 live inputs and 3 live outputs of the moment network
 # (2 (20 x 128 + 128 x 128 + 128 x 128 + 128 x 3) per row); the rest is padding to MFMA tiles.
 USEFUL_MLP_FLOP_PER_ROW = 2 * (20 * 128 + 128 * 128 + 128 * 128 + 128 * 3)
+# MFMA work of one training step per batch row: the forward pass, the four weight gradients
+# (the same products, reduced over rows) and the input gradients of layers 3, 2 and 1
+# (layer 0's input gradient is not needed): 2 MLP_FLOP_PER_ROW + 2 (32 x 128 + 2 x 128 x 128).
+TRAIN_FLOP_PER_ROW = 2 * MLP_FLOP_PER_ROW + 2 * (32 * 128 + 128 * 128 + 128 * 128)
 MACRO = 4
 DEFAULT_THRESHOLD = 0.01
 # Rollout policy of the measured configuration, chosen on unseen dam columns of the training
@@ -331,13 +336,14 @@ def perturb(solver: FluidSolver, spacing: float, pos_noise: float, vel_noise: fl
 
 def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4096, dt: float = 1.0 / 600.0,
              gap_steps: int = 2, seed: int = 2024, first: int = 0, noise=NOISE, net: "SurrogateNet | None" = None,
-             dam_every: int = 3, dam_warmup: int = 48, log=None) -> list[tuple[int, int]]:
+             dam_every: int = 3, dam_warmup: int = 48, log=None, solver_flags: int = 0) -> list[tuple[int, int]]:
     """Records `records` macro-steps of each of scenes first..first+scenes-1 into `data`;
     returns each scene's row range. Between records the scene runs `gap_steps` steps so
     the rows of one scene are not all alike; before record r the state is perturbed by
     noise[r % len(noise)]. With `net`, the gap steps are rollout steps of that network
     with the check off, so the rows come from the states the surrogate itself reaches
-    (data aggregation, Ross et al. 2011) with the plain path's response as target."""
+    (data aggregation, Ross et al. 2011) with the plain path's response as target.
+    `solver_flags` are SPH_CFG_* flags of the recording solver (e.g. CFG_NO_GRAPH)."""
     ranges = []
     solver = None
     cap = 0
@@ -349,7 +355,7 @@ def generate(data: TrainingSet, scenes: int, records: int = 12, samples: int = 4
             if solver is not None:
                 solver.close()
             cap = max(n, 65536)
-            solver = FluidSolver(cap, max(sc.workload.cell_count, 1 << 18), keep_forces=True)
+            solver = FluidSolver(cap, max(sc.workload.cell_count, 1 << 18), keep_forces=True, flags=solver_flags)
         solver.load_state(pos, vel)
         sub = fluid_substeps(p, dt)
         dt_sub = min(dt, 1.0 / 30.0) / sub
@@ -384,13 +390,14 @@ def knn_weights() -> Path:
 
 
 def record_workload(data: TrainingSet, workload: Workload, records: int, samples: int, warmup_steps: int = 0,
-                    gap_steps: int = 2, dt: float = 1.0 / 600.0, seed: int = 99) -> tuple[int, int]:
+                    gap_steps: int = 2, dt: float = 1.0 / 600.0, seed: int = 99,
+                    solver_flags: int = 0) -> tuple[int, int]:
     """Rows recorded from any workload's plain trajectory (e.g. the dam breaks the
     rollout is measured on), for evaluating the network on that distribution."""
     p = workload.params()
     pos, vel = workload.initial_state(p)
     n = pos.shape[0]
-    solver = FluidSolver(n, workload.cell_count, keep_forces=True)
+    solver = FluidSolver(n, workload.cell_count, keep_forces=True, flags=solver_flags)
     solver.load_state(pos, vel)
     sub = fluid_substeps(p, dt)
     dt_sub = float(np.float32(np.float32(min(dt, 1.0 / 30.0)) / np.float32(sub)))
@@ -408,30 +415,40 @@ def record_workload(data: TrainingSet, workload: Workload, records: int, samples
 def train_default(net: SurrogateNet, scenes: int = 200, held_out: int = 20, records: int = 12,
                   samples: int = 4096, steps: int = 6000, batch: int = 8192, lr: float = 1e-3,
                   eval_workloads=(), dagger_rounds: int = 0, dam_every: int = 3, dam_warmup: int = 48,
-                  log=sys.stderr) -> dict:
-    """(The rows are the network's feature kind.)"""
+                  log=sys.stderr, solver_flags: int = 0) -> dict:
     """The training recipe: `scenes` trajectories, the last `held_out` kept for evaluation;
     Adam at lr for `steps` batches, then lr / 10 for a quarter as many. Each of
     `dagger_rounds` rounds then adds the same number of scenes recorded along the
     network's own rollouts (seeds not used before) and trains again on all rows. Each
     workload in `eval_workloads` contributes 8 records of 16384 rows from its own plain
-    trajectory, used for evaluation only."""
+    trajectory, used for evaluation only. The rows are of the network's feature kind."""
     per_round = scenes * records * samples
     data = TrainingSet(per_round * (1 + dagger_rounds) + len(eval_workloads) * 8 * 16384,
                        feature_kind=net.feature_kind)
-    ranges = generate(data, scenes, records, samples, dam_every=dam_every, dam_warmup=dam_warmup, log=log)
+    ranges = generate(data, scenes, records, samples, dam_every=dam_every, dam_warmup=dam_warmup, log=log,
+                      solver_flags=solver_flags)
     split = ranges[scenes - held_out][0] if held_out else len(data)
     end = len(data)
-    evals = [(wl.name, record_workload(data, wl, 8, 16384, warmup_steps=4, gap_steps=4)) for wl in eval_workloads]
+    evals = [(wl.name, record_workload(data, wl, 8, 16384, warmup_steps=4, gap_steps=4, solver_flags=solver_flags))
+             for wl in eval_workloads]
+    t0 = time.perf_counter()
     losses = net.train(data, 0, split, batch, steps, lr, losses=True)
     tail = net.train(data, 0, split, batch, max(1, steps // 4), lr * 0.1, seed=11, losses=True)
+    seconds = time.perf_counter() - t0   # every training step ends in a synchronisation
     report = {"rows": len(data), "train_rows": split, "loss_first": float(losses[0]),
               "loss_last": float(tail[-1])}
+    # Whole training steps (gather, products, reductions, Adam and the per-step loss
+    # readback) against the MFMA work of the padded batch.
+    train_steps = steps + max(1, steps // 4)
+    rows_padded = (batch + 31) // 32 * 32
+    report["train_steps"] = train_steps
+    report["train_ms_per_step"] = seconds / train_steps * 1e3
+    report["train_tflops"] = TRAIN_FLOP_PER_ROW * rows_padded * train_steps / seconds / 1e12
     train_ranges = [(0, split)]
     for rnd in range(dagger_rounds):
         b0 = len(data)
         generate(data, scenes, records, samples, first=scenes * (rnd + 1), net=net, dam_every=dam_every,
-                 dam_warmup=dam_warmup, log=log)
+                 dam_warmup=dam_warmup, log=log, solver_flags=solver_flags)
         train_ranges.append((b0, len(data)))
         # rows of all rounds: train on a window ending at the newest rows and covering them all
         for k, (r0, r1) in enumerate(train_ranges):
@@ -490,7 +507,6 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     the surrogate from the same initial state and returns timing and density/energy
     statistics. The timed runs carry no statistics work; the statistics come from
     separate, untimed runs."""
-    import time
     p = workload.params()
     pos0, vel0 = workload.initial_state(p)
     n = pos0.shape[0]
@@ -635,6 +651,8 @@ def main(argv=None) -> int:
                    help="dam breaks whose own trajectories the network is also evaluated on")
     t.add_argument("--features", choices=("moments", "knn8"), default="moments",
                    help="network input: neighbourhood moments or the 8 nearest neighbours")
+    t.add_argument("--eager", action="store_true",
+                   help="record with kernels launched one by one instead of replayed step graphs")
     e = sp.add_parser("compare", help="plain path against the surrogate on a dam break")
     e.add_argument("--weights", default=str(default_weights()))
     e.add_argument("--workload", choices=("m1", "r4m", "s16k"), default="m1")
@@ -668,14 +686,14 @@ def main(argv=None) -> int:
                           "mean_speedup": sum(speedups) / max(1, len(speedups))}), flush=True)
         return 0
     if a.cmd == "train":
-        import time
         net = SurrogateNet(seed=a.seed)
         net.feature_kind = capi.FEATURES_KNN8 if a.features == "knn8" else capi.FEATURES_MOMENTS
         t0 = time.perf_counter()
         from . import fluid
         wls = [{"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[w]() for w in a.eval]
         rep = train_default(net, a.scenes, a.held_out, a.records, a.samples, a.steps, a.batch, a.lr, wls,
-                            a.dagger, a.dam_every, a.dam_warmup)
+                            a.dagger, a.dam_every, a.dam_warmup,
+                            solver_flags=capi.CFG_NO_GRAPH if a.eager else 0)
         rep["seconds"] = time.perf_counter() - t0
         Path(a.out).parent.mkdir(parents=True, exist_ok=True)
         net.save(a.out)
