@@ -521,6 +521,8 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
 
     probe = None
     step_errors = []
+    occupancy = [0]   # fullest spatial bin seen at the untimed runs' progress points
+    done = [0]        # steps the current run has completed
 
     def run(use_net: bool, timed: bool, label: str, errors: bool = False):
         nonlocal threshold, probe
@@ -530,7 +532,9 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
         series = []
         solver.synchronize()
         t0 = time.perf_counter()
+        done[0] = 0
         for i in range(steps):
+            done[0] = i
             if use_net and errors and (i + 1) % error_every == 0:
                 # this step and the plain step from the same state, compared particle by particle
                 if probe is None:
@@ -545,7 +549,7 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
             # progress of the untimed runs (the host waits for the device here, so a slow run
             # shows its pace instead of going quiet); timed runs report once at their end
             if not timed and (i + 1) % progress_every == 0:
-                solver.synchronize()
+                occupancy[0] = max(occupancy[0], solver.max_bin_occupancy())
                 say(f"{label}: step {i + 1}/{steps}, {time.perf_counter() - t0:.1f} s")
         solver.synchronize()
         return time.perf_counter() - t0, st, series
@@ -562,14 +566,24 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
         # the same rollout with the check never failing: the cost of the surrogate alone
         saved = threshold
         threshold = float("inf")
-        t_free, _, _ = run(True, True, "surrogate, check off")
-        say(f"surrogate, check off: {1e3 * t_free / steps:.3f} ms/step")
-        net.inference_time()
-        result["unchecked_ms_per_step"] = 1e3 * t_free / steps
-        result["unchecked_speedup"] = t_plain / t_free
-        _, _, free_series = run(True, False, "surrogate statistics, check off")
-        result["unchecked_density_excess"] = [float(v[0]) for v in free_series]
-        threshold = saved
+        try:
+            t_free, _, _ = run(True, True, "surrogate, check off")
+            say(f"surrogate, check off: {1e3 * t_free / steps:.3f} ms/step")
+            result["unchecked_ms_per_step"] = 1e3 * t_free / steps
+            result["unchecked_speedup"] = t_plain / t_free
+            _, _, free_series = run(True, False, "surrogate statistics, check off")
+            result["unchecked_density_excess"] = [float(v[0]) for v in free_series]
+        except capi.SphError as e:
+            if e.status != capi.ERR_DEGENERATE:
+                raise
+            # unchecked, the network's trajectory collapsed: the solver refused the state
+            result["unchecked_degenerate_at_step"] = done[0] + 1
+            result["unchecked_degenerate"] = str(e)
+            say(f"surrogate, check off: refused at step {done[0] + 1}: {e}")
+        finally:
+            threshold = saved
+            net.inference_time()
+    result["max_bin_occupancy"] = occupancy[0]
     result["plain_ms_per_step"] = 1e3 * t_plain / steps
     result["surrogate_ms_per_step"] = 1e3 * t_sur / steps
     result["plain_particle_steps_per_s"] = n * steps / t_plain
@@ -661,6 +675,11 @@ def main(argv=None) -> int:
     e.add_argument("--json", default=None)
     e.add_argument("--smoothing", type=float, default=DEFAULT_SMOOTHING)
     e.add_argument("--backoff", type=int, default=DEFAULT_BACKOFF)
+    e.add_argument("--no-unchecked", action="store_true", help="skip the runs with the check switched off")
+    e.add_argument("--stats-every", type=int, default=1, help="density statistics every this many steps")
+    e.add_argument("--error-every", type=int, default=5,
+                   help="per-particle error against plain from the same state every this many steps")
+    e.add_argument("--progress-every", type=int, default=20)
     v = sp.add_parser("validate", help="checked rollouts of held-out dam columns of the training family")
     v.add_argument("--weights", default=str(default_weights()))
     v.add_argument("--scenes", type=int, nargs="*", default=list(VALIDATION_SCENES))
@@ -705,7 +724,8 @@ def main(argv=None) -> int:
     net.set_smoothing(a.smoothing)
     net.set_backoff(a.backoff)
     wl = {"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[a.workload]()
-    res = compare(wl, net, a.steps, threshold=a.threshold, log=sys.stderr)
+    res = compare(wl, net, a.steps, threshold=a.threshold, stats_every=a.stats_every, unchecked=not a.no_unchecked,
+                  log=sys.stderr, progress_every=a.progress_every, error_every=a.error_every)
     if a.json:
         Path(a.json).write_text(json.dumps(res))
     res.pop("series_plain")
