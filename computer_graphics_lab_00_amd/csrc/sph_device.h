@@ -131,8 +131,8 @@ struct GridBuffers {
     // For the tiled density pass (set when the launch has half-tiles): the list of particles
     // whose own cell lies outside the half-tile they are sorted into.
     uint2* rest = nullptr;           // (slot, scene)
-    // When set: the largest bin occupancy of the build (zeroed with the counts, raised by
-    // k_bin_slots), and its copy in mapped host memory, written by k_rank_reorder.
+    // When set: each k_bin_slots workgroup's largest bin occupancy (one per workgroup of
+    // the launch), and the largest of them in mapped host memory, written by k_rank_reorder.
     uint32_t* binMax = nullptr;
     uint32_t* binMaxHost = nullptr;
 };

@@ -146,7 +146,7 @@ private:
     uint32_t maxTiles = 0, planTiles = 0;
     uint2* restList = nullptr;        // particles the tiled density pass leaves to its global walk
     uint32_t* restCount = nullptr;
-    uint32_t* binMax = nullptr;         // device word: largest bin occupancy of the current build
+    uint32_t* binMax = nullptr;         // per bin-slot workgroup: its largest bin occupancy
     uint32_t* binMaxHost = nullptr;     // its copy in mapped host memory (host address)
     uint32_t* binMaxHostDev = nullptr;  // the same memory's device address
     void* hostStage = nullptr;

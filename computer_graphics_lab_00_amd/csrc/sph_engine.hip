@@ -311,7 +311,7 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
     hostStageBytes = kMaxSubsteps * S * sizeof(sphdev::SceneDesc) + S * sizeof(sphdev::BoundaryConsts) +
                      (maxBlocks + 1) * sizeof(uint16_t) + ((size_t)maxTiles + 1) * sizeof(uint16_t) + 512;
     if (e == hipSuccess) e = hipHostMalloc(&hostStage, hostStageBytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&binMax, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&binMax, (maxBlocks + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(&binMaxHost, sizeof(uint32_t), hipHostMallocMapped);
     if (e == hipSuccess) {
         *binMaxHost = 0u;

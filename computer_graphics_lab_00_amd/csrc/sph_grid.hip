@@ -77,11 +77,9 @@ __device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ul
 // counts = 0 and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
 __global__ void __launch_bounds__(kBlock)
     k_grid_clear(uint32_t* __restrict__ counts, uint32_t* __restrict__ rep, uint32_t cells,
-                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount,
-                 uint32_t* __restrict__ binMax) {
+                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount) {
     const uint32_t stride = gridDim.x * kBlock;
     if (restCount && blockIdx.x == 0 && threadIdx.x == 0) *restCount = 0u;
-    if (binMax && blockIdx.x == 0 && threadIdx.x == 0) *binMax = 0u;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride) {
         counts[i] = 0u;
         rep[i] = 0xFFFFFFFFu;
@@ -140,6 +138,22 @@ __global__ void __launch_bounds__(kBlock)
     if (w.valid && (head || prevBin != bin)) atomicMin(&rep[key], bin);
 }
 
+// Maximum over the workgroup, returned to every lane (every lane of the workgroup must call it).
+__device__ inline uint32_t block_max(uint32_t m) {
+    __shared__ uint32_t waveMax[kBlock / 64];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(m, d, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63u) == 0u) waveMax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = waveMax[0];
+#pragma unroll
+    for (uint32_t w = 1; w < kBlock / 64; ++w) m = waveMax[w] > m ? waveMax[w] : m;
+    return m;
+}
+
 __global__ void __launch_bounds__(kBlock)
     k_bin_slots(Launch L, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rep,
                 uint32_t* __restrict__ binCount, uint32_t* __restrict__ binOf, uint32_t* __restrict__ slots,
@@ -168,16 +182,12 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (binMax) {
         // The largest bin's last member gets slot (occupancy - 1): the maximum of slot + 1
-        // over all particles is the largest bin occupancy. One atomic per wave at most, and
-        // none once the running maximum is at least this wave's.
+        // over all particles is the largest bin occupancy. Each workgroup stores its own
+        // maximum (no atomics: one contended address is serialised across the XCDs);
+        // k_rank_reorder reduces them.
         uint32_t m = w.valid ? slot + 1u : 0u;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            const uint32_t o = __shfl_xor(m, d, 64);
-            m = o > m ? o : m;
-        }
-        if (lane == 0u && m > __hip_atomic_load(binMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(binMax, m);
+        m = block_max(m);
+        if (threadIdx.x == 0) binMax[blockIdx.x] = m;
     }
 }
 
@@ -283,10 +293,14 @@ __global__ void __launch_bounds__(kBlock)
                    uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart, uint2* __restrict__ rest,
                    uint32_t* __restrict__ restCount, const uint32_t* __restrict__ binMax,
                    uint32_t* __restrict__ binMaxHost) {
-    // The largest bin occupancy of this build (final: k_bin_slots has completed) to the
-    // host's mapped copy, by one lane.
-    if (binMaxHost && blockIdx.x == 0 && threadIdx.x == 0)
-        *binMaxHost = __hip_atomic_load(binMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Workgroup 0 reduces k_bin_slots's per-workgroup maxima (final: that launch has
+    // completed) into the largest bin occupancy of this build, for the host's mapped copy.
+    if (binMaxHost && blockIdx.x == 0) {
+        uint32_t m = 0;
+        for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) m = binMax[b] > m ? binMax[b] : m;
+        m = block_max(m);
+        if (threadIdx.x == 0) *binMaxHost = m;
+    }
     const Where w = locate(L);
     const uint2 me = w.valid ? pairKI[w.slot] : make_uint2(0u, 0u);
     const uint32_t src = w.valid ? pairSrc[w.slot] : 0u;
@@ -407,8 +421,7 @@ hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t 
     const uint32_t span = L.totalCells > L.totalBins ? L.totalCells : L.totalBins;
     uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
-    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount,
-                                           b.binMax);
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount);
     return hipGetLastError();
 }
 
