@@ -48,7 +48,7 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 hipError_t launch_reduce_partials(const float* partial, uint32_t splits, uint32_t count, float* out, hipStream_t s);
 hipError_t launch_mse_grad(const float* y, const float* target, uint32_t rows, uint32_t rowsPadded, uint32_t ld,
                            uint32_t outs, float scale, uint16_t* dy, float* rowErr, hipStream_t s);
-// Column sums over `rows` rows; `partial` holds ceil(rows / 256) * cols floats.
+// Column sums over `rows` rows; `partial` holds (c + ceil(c / 16)) * cols floats, c = ceil(rows / 64).
 hipError_t launch_column_sum_bf16(const uint16_t* m, uint32_t rows, uint32_t cols, float* partial, float* out,
                                   hipStream_t s);
 hipError_t launch_adam(float* w, float* m1, float* m2, const float* grad, uint32_t count, float lr, float b1,

@@ -86,13 +86,22 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     }
 }
 
-// Sums split-K partials in chunk order (deterministic): out[i] = sum_c partial[c][i].
+// Sums split-K partials in chunk order (deterministic): out[i] = sum_c partial[c][i]. Eight
+// loads are issued before their ordered adds, so the chain waits on memory once per eight.
 __global__ void __launch_bounds__(256)
     k_reduce_partials(const float* __restrict__ partial, uint32_t splits, uint32_t count, float* __restrict__ out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= count) return;
     float s = 0.0f;
-    for (uint32_t c = 0; c < splits; ++c) s += partial[(size_t)c * count + i];
+    uint32_t c = 0;
+    for (; c + 8u <= splits; c += 8u) {
+        float v[8];
+#pragma unroll
+        for (uint32_t e = 0; e < 8u; ++e) v[e] = partial[(size_t)(c + e) * count + i];
+#pragma unroll
+        for (uint32_t e = 0; e < 8u; ++e) s += v[e];
+    }
+    for (; c < splits; ++c) s += partial[(size_t)c * count + i];
     out[i] = s;
 }
 
@@ -116,15 +125,41 @@ __global__ void __launch_bounds__(256)
     if (rowErr) rowErr[m] = e;
 }
 
-// Column sums of a row-major BF16 matrix over chunks of 256 rows: partial[chunk][col],
+// First stage of a two-stage ordered sum of `chunks` partial rows of `cols` values:
+// out[g][c] = sum over the g-th run of `per` consecutive chunks, in chunk order.
+__global__ void __launch_bounds__(256)
+    k_reduce_runs(const float* __restrict__ partial, uint32_t chunks, uint32_t per, uint32_t cols,
+                  float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t runs = (chunks + per - 1u) / per;
+    if (i >= runs * cols) return;
+    const uint32_t g = i / cols, c = i % cols;
+    const uint32_t k0 = g * per, k1 = k0 + per < chunks ? k0 + per : chunks;
+    float s = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) s += partial[(size_t)k * cols + c];
+    out[i] = s;
+}
+
+// Column sums of a row-major BF16 matrix over chunks of kColumnChunk rows: partial[chunk][col],
 // rows in order inside a chunk; chunks are then summed in order by k_reduce_partials
-// (deterministic). The bias gradient.
+// (deterministic). The bias gradient. Short chunks give the launch enough workgroups to
+// spread over the GPU; eight rows are loaded before their ordered adds.
+constexpr uint32_t kColumnChunk = 64;
+
 __global__ void __launch_bounds__(256)
     k_column_partial_bf16(const uint16_t* __restrict__ m, uint32_t rows, uint32_t cols, float* __restrict__ partial) {
-    const uint32_t r0 = blockIdx.x * 256u, r1 = r0 + 256u < rows ? r0 + 256u : rows;
+    const uint32_t r0 = blockIdx.x * kColumnChunk, r1 = r0 + kColumnChunk < rows ? r0 + kColumnChunk : rows;
     for (uint32_t c = threadIdx.x; c < cols; c += 256u) {
         float s = 0.0f;
-        for (uint32_t r = r0; r < r1; ++r) s += bf16_bits_to_f32(m[(size_t)r * cols + c]);
+        uint32_t r = r0;
+        for (; r + 8u <= r1; r += 8u) {
+            float v[8];
+#pragma unroll
+            for (uint32_t e = 0; e < 8u; ++e) v[e] = bf16_bits_to_f32(m[(size_t)(r + e) * cols + c]);
+#pragma unroll
+            for (uint32_t e = 0; e < 8u; ++e) s += v[e];
+        }
+        for (; r < r1; ++r) s += bf16_bits_to_f32(m[(size_t)r * cols + c]);
         partial[(size_t)blockIdx.x * cols + c] = s;
     }
 }
@@ -178,10 +213,15 @@ hipError_t launch_mse_grad(const float* y, const float* target, uint32_t rows, u
 
 hipError_t launch_column_sum_bf16(const uint16_t* m, uint32_t rows, uint32_t cols, float* partial, float* out,
                                   hipStream_t s) {
-    const uint32_t chunks = (rows + 255u) / 256u;
+    const uint32_t chunks = (rows + kColumnChunk - 1u) / kColumnChunk;
     if (chunks == 0) return hipErrorInvalidValue;
     k_column_partial_bf16<<<chunks, 256, 0, s>>>(m, rows, cols, partial);
-    k_reduce_partials<<<(cols + 255u) / 256u, 256, 0, s>>>(partial, chunks, cols, out);
+    // runs of 16 chunks summed in order into the space after the partials, then the runs in order
+    constexpr uint32_t kPer = 16;
+    const uint32_t runs = (chunks + kPer - 1u) / kPer;
+    float* stage = partial + (size_t)chunks * cols;
+    k_reduce_runs<<<(runs * cols + 255u) / 256u, 256, 0, s>>>(partial, chunks, kPer, cols, stage);
+    k_reduce_partials<<<(cols + 255u) / 256u, 256, 0, s>>>(stage, runs, cols, out);
     return hipGetLastError();
 }
 

@@ -12,10 +12,13 @@ import sqlite3
 
 
 def _short(name: str) -> str:
-    m = re.search(r"(k_[a-z_0-9]+)", name)
-    key = m.group(1) if m else name[:60]
+    """The kernel's name with its template arguments (demangled or mangled symbol)."""
+    m = re.search(r"(k_[a-z_0-9]+)(<[^>()]*>)?", name)
+    if not m:
+        return name[:60]
+    key = m.group(1) + (m.group(2).replace(" ", "") if m.group(2) else "")
     t = re.search(r"k_[a-z_0-9]+I((?:Lb\d)+)E", name)
-    if t:
+    if t and not m.group(2):
         key += "<" + ",".join(x for x in re.findall(r"Lb(\d)", t.group(1))) + ">"
     return key
 
