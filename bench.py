@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bucket-walk", action="store_true",
                     help="force pass walks the 27 buckets instead of the density pass's neighbour list")
+    ap.add_argument("--tiled-density", action="store_true",
+                    help="the LDS-tiled density pass (SPH_CFG_TILED_DENSITY) instead of the default one")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one 256 MiB device copy kernel first (byte-counter calibration)")
     ap.add_argument("--workload", default="dam_break_1m", choices=["dam_break_1m", "batch64"],
@@ -345,7 +347,7 @@ def main():
     print(f"[bench] workload {w.name}: {n} particles, {p.cellCount} buckets, {substeps} substeps per step",
           file=sys.stderr, flush=True)
 
-    flags = capi.CFG_BUCKET_WALK if args.bucket_walk else 0
+    flags = (capi.CFG_BUCKET_WALK if args.bucket_walk else 0) | (capi.CFG_TILED_DENSITY if args.tiled_density else 0)
     gpu = fluid.FluidSolver(n, max(p.cellCount, 1024), flags=flags)
     gpu.load_state(pos, vel)
     for _ in range(args.warmup):
@@ -449,7 +451,7 @@ def main():
         "config": {"workload": w.name, "particles": n, "lattice": list(w.lattice), "spacing": w.spacing,
                    "smoothing_radius": w.smoothing, "cell_count": p.cellCount, "substeps_per_step": substeps,
                    "dt": DT, "eos": "linear k=2000", "mode": "reference (32-slot cap, duplicate visits)",
-                   "neighbour_list": not args.bucket_walk,
+                   "neighbour_list": not args.bucket_walk, "tiled_density": bool(args.tiled_density),
                    "particle_substeps_per_s": value * substeps},
         "roofline": roofline,
         "cpu_baseline": cpu,
