@@ -58,19 +58,36 @@ __device__ __forceinline__ __bf16 bits_bf16(uint16_t b) {
 
 __device__ __forceinline__ bf16x8 lds8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
-// Bias, ReLU and BF16 rounding of one 32 x 32 accumulator tile into two B operands.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
+// Two values rounded to BF16 in one packed word, then ReLU on their bit patterns as signed
+// 16-bit integers: every negative value (and -0) has the sign bit set, so max(bits, 0) maps
+// it to +0 and keeps the others. Rounding to BF16 is monotone and keeps the sign, so ReLU
+// after rounding equals rounding after ReLU.
+__device__ __forceinline__ uint32_t relu_pair_bf16(float a, float b) {
+    const bf16x2 c = __builtin_convertvector((f32x2){a, b}, bf16x2);
+    i16x2 x;
+    __builtin_memcpy(&x, &c, sizeof(x));
+    x = __builtin_elementwise_max(x, (i16x2)0);
+    uint32_t w;
+    __builtin_memcpy(&w, &x, sizeof(w));
+    return w;
+}
+
+// Bias, BF16 rounding and ReLU of one 32 x 32 accumulator tile into two B operands.
 __device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias, uint32_t half, bf16x8& lo,
                                             bf16x8& hi) {
+    uint32_t w[8];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const uint32_t f = 8u * (uint32_t)(j / 4) + 4u * half + (uint32_t)(j % 4);
-        float v = acc[j] + bias[f];
-        v = v > 0.0f ? v : 0.0f;
-        if (j < 8)
-            lo[j] = (__bf16)v;
-        else
-            hi[j - 8] = (__bf16)v;
+    for (int p = 0; p < 8; ++p) {
+        const int j = 2 * p;
+        const uint32_t f = 8u * (uint32_t)(j / 4) + 4u * half + (uint32_t)(j % 4);   // j, j + 1: f, f + 1
+        w[p] = relu_pair_bf16(acc[j] + bias[f], acc[j + 1] + bias[f + 1]);
     }
+    __builtin_memcpy(&lo, &w[0], sizeof(lo));
+    __builtin_memcpy(&hi, &w[4], sizeof(hi));
 }
 
 // One layer over the first KB k-blocks of its (LD-strided) staged weights.
