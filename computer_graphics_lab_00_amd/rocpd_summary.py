@@ -17,9 +17,9 @@ def _short(name: str) -> str:
     if not m:
         return name[:60]
     key = m.group(1) + (m.group(2).replace(" ", "") if m.group(2) else "")
-    t = re.search(r"k_[a-z_0-9]+I((?:Lb\d)+)E", name)
+    t = re.search(r"k_[a-z_0-9]+I((?:Lb\dE)+)E", name)   # mangled bool arguments: I Lb1E Lb0E ... E
     if t and not m.group(2):
-        key += "<" + ",".join(x for x in re.findall(r"Lb(\d)", t.group(1))) + ">"
+        key += "<" + ",".join(x for x in re.findall(r"Lb(\d)E", t.group(1))) + ">"
     return key
 
 

@@ -33,6 +33,39 @@ def test_library_exports_every_declared_symbol(sph):
     assert not missing, f"declared but not exported: {sorted(missing)}"
 
 
+def _header_constants():
+    """Every `#define SPH_X <integer>u` and every `SPH_X = <integer>` enumerator of the header."""
+    text = (ROOT / "include" / "sph_c_api.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define\s+(SPH_[A-Z0-9_]+)\s+(\d+)u?\b", text)}
+    out.update({m.group(1): int(m.group(2)) for m in re.finditer(r"\b(SPH_[A-Z0-9_]+)\s*=\s*(\d+)", text)})
+    return out
+
+
+def test_python_mirror_constants_equal_the_header():
+    """The ctypes mirror's statuses, flags, modes and limits are the header's values."""
+    h = _header_constants()
+    for code, name in capi.STATUS_NAMES.items():
+        assert h[name] == code, name
+    assert max(capi.STATUS_NAMES) == max(v for k, v in h.items() if k.startswith("SPH_ERR_"))
+    pairs = {
+        "SPH_MAX_COLLIDERS": capi.SPH_MAX_COLLIDERS, "SPH_MAX_FORCE_FIELDS": capi.SPH_MAX_FORCE_FIELDS,
+        "SPH_CELL_CAPACITY": capi.SPH_CELL_CAPACITY, "SPH_MAX_BIN_OCCUPANCY": capi.MAX_BIN_OCCUPANCY,
+        "SPH_ERR_DEGENERATE": capi.ERR_DEGENERATE,
+        "SPH_CFG_KEEP_FORCES": capi.CFG_KEEP_FORCES, "SPH_CFG_NO_GRAPH": capi.CFG_NO_GRAPH,
+        "SPH_CFG_BUCKET_WALK": capi.CFG_BUCKET_WALK, "SPH_CFG_TILED_DENSITY": capi.CFG_TILED_DENSITY,
+        "SPH_MODE_REFERENCE": capi.MODE_REFERENCE, "SPH_MODE_UNCAPPED": capi.MODE_UNCAPPED,
+        "SPH_MODE_DEDUPE": capi.MODE_DEDUPE, "SPH_MODE_FAST_MATH": capi.MODE_FAST_MATH,
+        "SPH_EOS_LINEAR": capi.EOS_LINEAR, "SPH_EOS_TAIT": capi.EOS_TAIT,
+        "SPH_CHECK_COMPRESSION": capi.CHECK_COMPRESSION, "SPH_CHECK_RAREFACTION": capi.CHECK_RAREFACTION,
+        "SPH_FEATURES_MOMENTS": capi.FEATURES_MOMENTS, "SPH_FEATURES_KNN8": capi.FEATURES_KNN8,
+        "SPH_SHAPE_SPHERE": capi.SHAPE_SPHERE, "SPH_SHAPE_CAPSULE": capi.SHAPE_CAPSULE,
+        "SPH_SHAPE_MESH": capi.SHAPE_MESH, "SPH_FIELD_POINT": capi.FIELD_POINT,
+    }
+    for name, value in pairs.items():
+        assert h[name] == value, name
+
+
 def test_create_without_a_device_fails_loudly(sph):
     import os
     if os.path.exists("/dev/kfd"):
