@@ -242,6 +242,7 @@ def _declare(lib):
         "sph_debug_step_constants": (C.c_int, [C.POINTER(SphParams), C.c_float, _FP]),
         "sph_debug_copy": (C.c_int, [C.c_uint64]),
         "sph_debug_gemm": (C.c_int, [_HP, _HP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _FP]),
+        "sph_debug_forward": (C.c_int, [P, _HP, C.c_uint32, C.c_uint32, _HP, _FP]),
         "sph_mc_table": (C.c_int, [C.POINTER(C.c_int8)]),
         "sph_write_instances": (C.c_int, [P, C.POINTER(SphParams), C.POINTER(SphInstanceConfig), C.c_void_p,
                                           C.c_void_p]),

@@ -63,6 +63,10 @@ hipError_t launch_mlp_fused(const uint16_t* x, uint32_t rows, const uint16_t* wB
 // The same network pass with the rows built in the kernel from the prepared state of a
 // single scene at slot offset 0: pos (density in .w) and the neighbourhood sums
 // (kSumRows x total float4), for rows <= total particles.
+// The training forward pass in the same kernel: rows x 64 BF16 features -> the hidden
+// activations h1, h2, h3 (rows x 128 BF16, after ReLU) and the 32 outputs (rows x 32 FP32).
+hipError_t launch_mlp_fused_train(const uint16_t* x, uint32_t rows, const uint16_t* wBf16, const float* wF32,
+                                  uint16_t* h1, uint16_t* h2, uint16_t* h3, float* y32, hipStream_t s);
 hipError_t launch_mlp_fused_sums(const float4* pos, const float4* sums, uint32_t total, uint32_t rows, float rho0,
                                  float h, float invV, const uint16_t* wBf16, const float* wF32, float4* y,
                                  uint32_t blocks, hipStream_t s);

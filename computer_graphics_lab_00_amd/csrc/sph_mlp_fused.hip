@@ -76,9 +76,11 @@ __device__ __forceinline__ uint32_t relu_pair_bf16(float a, float b) {
     return w;
 }
 
-// Bias, BF16 rounding and ReLU of one 32 x 32 accumulator tile into two B operands.
+// Bias, BF16 rounding and ReLU of one 32 x 32 accumulator tile into two B operands. With
+// `store`, the tile's 16 values of this lane's row also go to store[0 .. 31] in feature order
+// (the tile's 32 features): words 2q and 2q + 1 are features 8 q + 4 half .. + 3.
 __device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias, uint32_t half, bf16x8& lo,
-                                            bf16x8& hi) {
+                                            bf16x8& hi, uint16_t* store = nullptr) {
     uint32_t w[8];
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
@@ -88,12 +90,19 @@ __device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias
     }
     __builtin_memcpy(&lo, &w[0], sizeof(lo));
     __builtin_memcpy(&hi, &w[4], sizeof(hi));
+    if (store) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<uint2*>(store + 8u * (uint32_t)q + 4u * half) = make_uint2(w[2 * q], w[2 * q + 1]);
+    }
 }
 
 // One layer over the first KB k-blocks of its (LD-strided) staged weights.
 template <int KB, uint32_t LD, int KIN>
 __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                             uint32_t lane, const bf16x8 (&in)[2][KIN], bf16x8 (&out)[2][8]) {
+                                             uint32_t lane, const bf16x8 (&in)[2][KIN], bf16x8 (&out)[2][8],
+                                             uint16_t* __restrict__ hOut = nullptr, uint32_t row0 = 0,
+                                             uint32_t rows = 0) {
     const uint32_t r = lane & 31u, half = lane >> 5;
     const uint32_t ld = LD;
     f32x16 acc[2][4];
@@ -112,9 +121,15 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, con
         }
     }
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
+    for (int nt = 0; nt < 2; ++nt) {
+        // with hOut: the activations of rows row0 .. row0 + 63, 128 BF16 per row
+        const uint32_t row = row0 + (uint32_t)nt * 32u + r;
+        uint16_t* dst = (hOut && row < rows) ? hOut + (size_t)row * 128u : nullptr;
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) to_operands(acc[nt][mt], bias + mt * 32, half, out[nt][2 * mt], out[nt][2 * mt + 1]);
+        for (int mt = 0; mt < 4; ++mt)
+            to_operands(acc[nt][mt], bias + mt * 32, half, out[nt][2 * mt], out[nt][2 * mt + 1],
+                        dst ? dst + mt * 32 : nullptr);
+    }
 }
 
 struct FusedInput {
@@ -123,9 +138,14 @@ struct FusedInput {
     const float4* sums;    // its neighbourhood sums (kSumRows x total), when SUMS
     uint32_t total;
     float rho0, h, invV;
+    uint16_t* h1;          // STORE: the three hidden activations (rows x 128 BF16) and
+    uint16_t* h2;
+    uint16_t* h3;
+    float* y32;            // the 32 outputs (rows x 32 FP32) the backward pass reads
 };
 
-template <bool SUMS>
+// STORE (without SUMS): the training forward pass, which also keeps every layer's output.
+template <bool SUMS, bool STORE = false>
 __global__ void __launch_bounds__(kThreads)
     k_mlp_fused(FusedInput in, uint32_t rows, const uint16_t* __restrict__ wb, const float* __restrict__ wf,
                 float4* __restrict__ y) {
@@ -196,9 +216,9 @@ __global__ void __launch_bounds__(kThreads)
             }
         }
         bf16x8 h1[2][8], h2[2][8];
-        hidden_layer<KB0, kLd0, KB0>(sw, sb, lane, in0, h1);
-        hidden_layer<8, kLdH, 8>(sw + kOff1, sb + 128, lane, h1, h2);
-        hidden_layer<8, kLdH, 8>(sw + kOff2, sb + 256, lane, h2, h1);
+        hidden_layer<KB0, kLd0, KB0>(sw, sb, lane, in0, h1, STORE ? in.h1 : nullptr, base, rows);
+        hidden_layer<8, kLdH, 8>(sw + kOff1, sb + 128, lane, h1, h2, STORE ? in.h2 : nullptr, base, rows);
+        hidden_layer<8, kLdH, 8>(sw + kOff2, sb + 256, lane, h2, h1, STORE ? in.h3 : nullptr, base, rows);
         // output layer: 32 (3 used) outputs, one tile per row group
         f32x16 acc[2] = {f32x16{}, f32x16{}};
 #pragma unroll
@@ -207,7 +227,21 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
             for (int nt = 0; nt < 2; ++nt) acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h1[nt][kb], acc[nt], 0, 0, 0);
         }
-        if (half == 0u) {   // elements 0..2 of the lower half are outputs 0..2
+        if (STORE) {   // all 32 outputs: element j of this lane is output 8 (j / 4) + 4 half + j % 4
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const uint32_t row = base + (uint32_t)nt * 32u + r;
+                if (row < rows) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t f = 8u * (uint32_t)q + 4u * half;
+                        *reinterpret_cast<float4*>(in.y32 + (size_t)row * 32u + f) =
+                            make_float4(acc[nt][4 * q] + sb[384 + f], acc[nt][4 * q + 1] + sb[385 + f],
+                                        acc[nt][4 * q + 2] + sb[386 + f], acc[nt][4 * q + 3] + sb[387 + f]);
+                    }
+                }
+            }
+        } else if (half == 0u) {   // elements 0..2 of the lower half are outputs 0..2
 #pragma unroll
             for (int nt = 0; nt < 2; ++nt) {
                 const uint32_t row = base + (uint32_t)nt * 32u + r;
@@ -237,6 +271,20 @@ hipError_t launch_mlp_fused(const uint16_t* x, uint32_t rows, const uint16_t* wB
     FusedInput in{};
     in.x = x;
     k_mlp_fused<false><<<blocks, kThreads, 0, s>>>(in, rows, wBf16, wF32, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_fused_train(const uint16_t* x, uint32_t rows, const uint16_t* wBf16, const float* wF32,
+                                  uint16_t* h1, uint16_t* h2, uint16_t* h3, float* y32, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    if (!x || !h1 || !h2 || !h3 || !y32) return hipErrorInvalidValue;
+    FusedInput in{};
+    in.x = x;
+    in.h1 = h1;
+    in.h2 = h2;
+    in.h3 = h3;
+    in.y32 = y32;
+    k_mlp_fused<false, true><<<mlp_fused_blocks(rows), kThreads, 0, s>>>(in, rows, wBf16, wF32, nullptr);
     return hipGetLastError();
 }
 

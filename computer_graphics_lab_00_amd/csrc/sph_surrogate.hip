@@ -207,8 +207,16 @@ sphmlp::GemmArgs gemm_forward(const uint16_t* x, const SphNet* n, const LayerDes
 }
 
 // X (rowsPad x 64, BF16) -> Y (rowsPad x 32, FP32); hidden activations into h1, h2, h3.
+// One launch of the fused network kernel (weights staged in LDS, all four layers per row group).
 int net_forward(SphNet* n, const uint16_t* x, uint32_t rowsPad, uint16_t* h1, uint16_t* h2, uint16_t* h3, float* y,
                 hipStream_t s) {
+    SPH_HIP(sphmlp::launch_mlp_fused_train(x, rowsPad, n->wb, n->w, h1, h2, h3, y, s));
+    return SPH_OK;
+}
+
+// The same forward pass by four tile GEMM launches (one per layer).
+int net_forward_gemm(SphNet* n, const uint16_t* x, uint32_t rowsPad, uint16_t* h1, uint16_t* h2, uint16_t* h3,
+                     float* y, hipStream_t s) {
     sphmlp::GemmArgs g = gemm_forward(x, n, kLayers[0], rowsPad);
     g.relu = 1;
     g.outBf16 = h1;
@@ -475,6 +483,27 @@ extern "C" int sph_net_gradients(SphNet* n, const uint16_t* x, const float* t, u
     SPH_HIP(hipMemcpyAsync(grad, n->grad, kParams * 4, hipMemcpyDeviceToHost, n->stream));
     SPH_HIP(hipStreamSynchronize(n->stream));
     if (loss) return net_loss(n, rows, loss);
+    return SPH_OK;
+}
+
+// The training forward pass on host rows, by the fused kernel (path 0) or the four tile GEMMs
+// (path 1): h gets the three hidden activations (3 x rows x 128 BF16 bits), y the 32 outputs
+// (rows x 32). A check that the two paths agree.
+extern "C" int sph_debug_forward(SphNet* n, const uint16_t* x, uint32_t rows, uint32_t path, uint16_t* h, float* y) {
+    sphhost::clear_error();
+    if (!n || !x || !h || !y || rows == 0 || path > 1) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_debug_forward");
+    hipSetDevice(n->device);
+    const uint32_t R = pad32(rows);
+    std::vector<float> t((size_t)rows * 3, 0.0f);
+    if (int st = upload_batch(n, x, t.data(), rows, R)) return st;
+    const int st = path == 0 ? net_forward(n, n->x, R, n->h[0], n->h[1], n->h[2], n->y, n->stream)
+                             : net_forward_gemm(n, n->x, R, n->h[0], n->h[1], n->h[2], n->y, n->stream);
+    if (st) return st;
+    for (int l = 0; l < 3; ++l)
+        SPH_HIP(hipMemcpyAsync(h + (size_t)l * rows * 128, n->h[l], (size_t)rows * 128 * 2, hipMemcpyDeviceToHost,
+                               n->stream));
+    SPH_HIP(hipMemcpyAsync(y, n->y, (size_t)rows * 32 * 4, hipMemcpyDeviceToHost, n->stream));
+    SPH_HIP(hipStreamSynchronize(n->stream));
     return SPH_OK;
 }
 

@@ -112,6 +112,48 @@ def test_forward_and_gradients_match_the_oracle(sph, rows):
     assert not np.any(g[SO.frozen_mask()])
 
 
+@pytest.mark.parametrize("rows", [1000, 4096])
+def test_fused_training_forward_matches_the_tile_gemms(sph, rows):
+    """The training forward pass by the fused kernel (one launch, weights in LDS, all four
+    layers per row group) against the four tile-GEMM launches. The fused kernel feeds each
+    16-wide k-block to the MFMA in a permuted order (so one layer's accumulators are the
+    next layer's operands in place), and the MFMA's sum depends on that order: FP32 results
+    agree within 1e-5 of the sum of the products' magnitudes (the MFMA layout test's bar),
+    the BF16 hidden activations, rounded from them, to one BF16 unit wherever a value sits on
+    a rounding boundary. The inference variant
+    equals the training variant bit for bit. 1000 rows leaves a partial row group."""
+    say(f"fused training forward rows={rows}")
+    net = surrogate.SurrogateNet(seed=7)
+    x, _ = _rows(rows, seed=rows + 1)
+    lib = capi.load()
+    hp = C.POINTER(C.c_uint16)
+    out = []
+    for path in (0, 1):
+        h = np.empty((3, rows, 128), np.uint16)
+        y = np.empty((rows, 32), np.float32)
+        capi.check(lib.sph_debug_forward(net._h, x.ctypes.data_as(hp), rows, path, h.ctypes.data_as(hp),
+                                         y.ctypes.data_as(C.POINTER(C.c_float))))
+        out.append((h, y))
+    (hf, yf), (hg, yg) = out
+    for layer in range(3):
+        steps = np.abs(hf[layer].astype(np.int32) - hg[layer].astype(np.int32))
+        flipped = np.count_nonzero(steps)
+        say(f"  hidden layer {layer + 1}: {flipped} of {steps.size} activations one BF16 unit apart")
+        assert steps.max() <= 1 and flipped <= steps.size // 1000
+    nin, nout, w, _ = SO.LAYERS[3]
+    w3 = np.abs(net.params[w: w + nout * nin].astype(np.float64).reshape(nout, nin))
+    scale = np.abs(SO.from_bits(hg[2]).astype(np.float64)) @ w3.T
+    # rows whose hidden activations agree exactly (a flipped activation carries its BF16 unit on)
+    same = np.all(hf == hg, axis=(0, 2))
+    assert np.count_nonzero(~same) <= rows // 100
+    err = np.abs(yf.astype(np.float64) - yg)[same] / np.maximum(scale[same], 1e-30)
+    say(f"  outputs: {np.count_nonzero(yf != yg)} of {yf.size} differ; over the {np.count_nonzero(same)} rows with "
+        f"equal activations max |diff| / sum |w h| = {err.max():.2e}")
+    assert err.max() < 1e-5
+    assert np.all(yf[:, surrogate.OUTPUTS:] == 0.0)   # padded outputs: zero weights and biases
+    assert np.array_equal(net.forward(x), yf[:, :surrogate.OUTPUTS])   # the inference variant
+
+
 def test_adam_first_step_matches_restatement(sph):
     say("adam first step")
     x, t = _rows(2048, seed=9)
