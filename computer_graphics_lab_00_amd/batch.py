@@ -154,6 +154,12 @@ class SceneBatch:
     def synchronize(self):
         check(self._lib.sph_batch_synchronize(self._h))
 
+    def max_bin_occupancy(self) -> int:
+        """Particles in the fullest spatial bin of the last grid build, over all scenes."""
+        out = C.c_uint32()
+        check(self._lib.sph_batch_max_bin_occupancy(self._h, C.byref(out)))
+        return int(out.value)
+
     def state(self, scene: int):
         n = self._lib.sph_batch_particle_count(self._h, scene)
         pos = np.empty((n, 4), np.float32)

@@ -235,6 +235,7 @@ def _declare(lib):
         "sph_batch_load_state": (C.c_int, [P, C.c_uint32, _FP, _FP, C.c_uint32]),
         "sph_batch_step": (C.c_int, [P, C.c_float]),
         "sph_batch_synchronize": (C.c_int, [P]),
+        "sph_batch_max_bin_occupancy": (C.c_int, [P, C.POINTER(C.c_uint32)]),
         "sph_batch_particle_count": (C.c_uint32, [P, C.c_uint32]),
         "sph_batch_read_particles": (C.c_int, [P, C.c_uint32, _FP, _FP]),
         "sph_batch_profile_enable": (C.c_int, [P, C.c_int]),

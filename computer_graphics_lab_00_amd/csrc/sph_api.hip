@@ -251,6 +251,14 @@ extern "C" int sph_batch_synchronize(SphBatch* b) {
     return b->e.synchronize();
 }
 
+extern "C" int sph_batch_max_bin_occupancy(SphBatch* b, uint32_t* out) {
+    sphhost::clear_error();
+    if (!b || !out) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_batch_max_bin_occupancy: null argument");
+    if (int st = b->e.synchronize()) return st;
+    *out = b->e.last_bin_max();
+    return SPH_OK;
+}
+
 extern "C" uint32_t sph_batch_particle_count(const SphBatch* b, uint32_t scene) {
     return (b && scene < b->e.sc.size()) ? b->e.sc[scene].n : 0u;
 }

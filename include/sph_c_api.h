@@ -319,6 +319,9 @@ int sph_batch_load_state(SphBatch* batch, uint32_t scene, const float* pos4, con
 /* FluidSolver::step for every scene; dt <= 0 is a no-op. Enqueued, returns early. */
 int sph_batch_step(SphBatch* batch, float deltaSeconds);
 int sph_batch_synchronize(SphBatch* batch);
+/* No counterpart in the reference: sph_max_bin_occupancy over every scene of the batch (a
+ * batch step returns SPH_ERR_DEGENERATE the same way). */
+int sph_batch_max_bin_occupancy(SphBatch* batch, uint32_t* out);
 uint32_t sph_batch_particle_count(const SphBatch* batch, uint32_t scene);
 int sph_batch_read_particles(SphBatch* batch, uint32_t scene, float* pos4, float* vel4);
 int sph_batch_profile_enable(SphBatch* batch, int enable);

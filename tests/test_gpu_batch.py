@@ -174,3 +174,21 @@ def test_batch64_at_config_size(oracle_lib):
         say(f"  scene {i}: {s.count} particles, {fluid.fluid_substeps(p, DT)} substeps")
         assert np.array_equal(bits(got[i][0]), bits(alone[0])) and np.array_equal(bits(got[i][1]), bits(alone[1]))
         assert np.array_equal(bits(got[i][0]), bits(op)) and np.array_equal(bits(got[i][1]), bits(ov))
+
+
+def test_batch_bin_occupancy_covers_every_scene():
+    """A scene hashed into one bucket (cellCount 1) puts all its particles into one spatial
+    bin: the batch's fullest bin is exactly that scene's particle count, beside a scene
+    hashed normally; a state loaded afterwards starts the figure over."""
+    specs = batch.batch64(13, scenes=2, n_min=2000, n_max=3000)
+    specs[1].cell_count = 1
+    say(f"case: batch occupancy, {specs[0].count} + {specs[1].count} particles (the second in one bucket)")
+    with batch.SceneBatch(specs) as b:
+        b.load_initial()
+        assert b.max_bin_occupancy() == 0
+        b.step(DT)
+        assert b.max_bin_occupancy() == specs[1].count
+        p = specs[0].params()
+        pos, vel = specs[0].initial_state(p)
+        b.load_state(0, pos, vel)
+        assert b.max_bin_occupancy() == 0
