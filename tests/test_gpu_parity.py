@@ -395,6 +395,20 @@ def test_m1_one_substep(flags, oracle_lib):
     gpu.close()
 
 
+def test_r4m_one_substep(oracle_lib):
+    """The largest configuration, 4,194,304 particles and 2^23 buckets (the surrogate
+    rollout's scene): grid (keys, counts, slot order) and one substep bit-exact."""
+    say("case: r4m one substep, cellCount 2^23")
+    w = fluid.r4m()
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    assert w.count == 1 << 22
+    dt = 1.0 / 600.0 / fluid.fluid_substeps(p, 1.0 / 600.0)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    compare_substep(gpu, ora, p, "r4m C=2^23")
+    gpu.close()
+
+
 def test_m1_clamped_cellcount_one_substep(oracle_lib):
     """The reference's own clamp gives 65536 buckets at 1M particles: 16 per bucket,
     so the 32-slot cap drops particles. Still bit-exact."""
