@@ -148,6 +148,10 @@ class SceneBatch:
         vel4 = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
         check(self._lib.sph_batch_load_state(self._h, scene, _fptr(pos4), _fptr(vel4), pos4.shape[0]))
 
+    def set_mode(self, scene: int, mode: int, eos: int = 0):
+        """The scene's neighbour-search / arithmetic mode (capi.MODE_*) and equation of state."""
+        check(self._lib.sph_batch_set_mode(self._h, scene, mode, eos))
+
     def step(self, dt: float):
         check(self._lib.sph_batch_step(self._h, dt))
 

@@ -192,3 +192,34 @@ def test_batch_bin_occupancy_covers_every_scene():
         pos, vel = specs[0].initial_state(p)
         b.load_state(0, pos, vel)
         assert b.max_bin_occupancy() == 0
+
+
+def test_scenes_of_different_modes_in_one_batch_equal_each_alone():
+    """A launch takes its kernel instances from the modes of all its scenes together (the
+    de-duplicating and fast-math instances carry a per-scene branch), so a reference-mode
+    scene can run inside another mode's instance. Each scene of a batch mixing the reference
+    mode, uncapped buckets, de-duplicated buckets with the Tait EOS and fast math equals the
+    same scene stepped alone in its own mode, bit for bit, over three steps."""
+    specs = batch.batch64(17, scenes=4, n_min=3000, n_max=9000)
+    modes = [(capi.MODE_REFERENCE, capi.EOS_LINEAR), (capi.MODE_UNCAPPED, capi.EOS_LINEAR),
+             (capi.MODE_DEDUPE, capi.EOS_TAIT), (capi.MODE_FAST_MATH, capi.EOS_LINEAR)]
+    say(f"case: mixed modes, {[s.count for s in specs]} particles")
+    with batch.SceneBatch(specs) as b:
+        for i, (mode, eos) in enumerate(modes):
+            b.set_mode(i, mode, eos)
+        b.load_initial()
+        for _ in range(3):
+            b.step(DT)
+        for i, (spec, (mode, eos)) in enumerate(zip(specs, modes)):
+            p = spec.params()
+            pos, vel = spec.initial_state(p)
+            with fluid.FluidSolver(spec.count, max(p.cellCount, 1024)) as s:
+                s.set_mode(mode)
+                s.set_eos(eos)
+                s.load_state(pos, vel)
+                for _ in range(3):
+                    s.step(p, DT)
+                ap, av = s.state()
+            bp, bv = b.state(i)
+            assert np.array_equal(bits(bp), bits(ap)), f"scene {i} (mode {mode}) positions differ"
+            assert np.array_equal(bits(bv), bits(av)), f"scene {i} (mode {mode}) velocities differ"
