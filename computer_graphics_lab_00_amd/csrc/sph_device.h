@@ -8,12 +8,13 @@
 //   pos4[slot]  float4 (x, y, z, density)     ping-pong pair
 //   vel4[slot]  float4 (vx, vy, vz, pressure) ping-pong pair
 //   ids[slot]   uint32 particle index within its scene, ping-pong pair
-//   counts[bucket], bucketStart[bucket]       hash buckets of all scenes, scene by scene
+//   buckets[bucket] uint2 (first slot, member count): hash buckets of all scenes, scene by
+//               scene; one 8-byte load gives a walk both numbers
 // Scene s owns slots [offset_s, offset_s + capacity_s), its particles the first n_s.
 // Inside a scene slots are ordered by space (spatial bin of the bucket), then bucket,
 // then particle index. Every bucket is contiguous and ascending in index — the order of
 // the reference's serial insertion (src/physics/fluid_sph.cpp:181-187) — so its
-// `cellParticles[b * 32 + k]` is `ids[bucketStart[b] + k]` for k < min(counts[b], 32).
+// `cellParticles[b * 32 + k]` is `ids[buckets[b].x + k]` for k < min(buckets[b].y, 32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -114,7 +115,8 @@ struct GridBuffers {
     const float4* vel;
     const uint32_t* ids;
     uint32_t* keys;         // global bucket per pre-sort slot
-    uint32_t* counts;       // totalCells
+    uint2* buckets;         // totalCells: (first slot, member count); the clear zeroes both, the
+                            // keys pass counts, the reorder sets the first slot of non-empty ones
     uint32_t* rep;          // totalCells: smallest global bin among each bucket's cells
     uint32_t* binStart;     // totalBins + 1: counts, then their exclusive scan
     uint32_t* binOf;        // per pre-sort slot
@@ -126,7 +128,6 @@ struct GridBuffers {
     float4* velSorted;
     uint32_t* idsSorted;
     uint32_t* keysSorted;
-    uint32_t* bucketStart;  // totalCells
     uint32_t* restCount = nullptr;   // when set: zeroed with the counts (the tiled density pass's list)
     // For the tiled density pass (set when the launch has half-tiles): the list of particles
     // whose own cell lies outside the half-tile they are sorted into.
@@ -208,11 +209,11 @@ struct DensityTiles {
 };
 
 // launchers (sph_forces.hip)
-hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
+hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2* buckets, uint4* nbrList,
+                          uint32_t* nbrCount,
                           const NeighbourSums* sums, hipStream_t s, const DensityTiles* tiles = nullptr);
-hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
-                        const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
+hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint2* buckets,
+                        const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s);
 
 }  // namespace sphdev

@@ -80,8 +80,7 @@ public:
     // Grid build only (keys, sort, bucket starts) on the current state: the sorted state
     // and the buckets describe the current positions. Nothing is committed.
     int build_grid();
-    const uint32_t* bucket_starts() const { return bucketStart; }
-    const uint32_t* bucket_counts() const { return counts; }
+    const uint2* bucket_table() const { return buckets; }   // (first slot, member count) per bucket
     float4* sorted_pos() { return pos[1 - pc]; }
     float4* sorted_vel() { return vel[1 - pc]; }
     uint32_t* sorted_ids() { return ids[1 - ic]; }
@@ -130,7 +129,8 @@ private:
     float4* vel[2] = {nullptr, nullptr};
     uint32_t* ids[2] = {nullptr, nullptr};
     int pc = 0, ic = 0;
-    uint32_t *keys = nullptr, *counts = nullptr, *rep = nullptr, *bucketStart = nullptr;
+    uint32_t *keys = nullptr, *rep = nullptr;
+    uint2* buckets = nullptr;   // (first slot, member count) per bucket
     uint32_t *binStart = nullptr, *binOf = nullptr, *slots = nullptr, *blockSums = nullptr, *keysSorted = nullptr;
     uint2* pairKI = nullptr;
     uint32_t* pairSrc = nullptr;

@@ -188,9 +188,8 @@ void Engine::free_all() {
         graph[i] = nullptr;
     }
     hipFree(keys);
-    hipFree(counts);
+    hipFree(buckets);
     hipFree(rep);
-    hipFree(bucketStart);
     hipFree(binStart);
     hipFree(binOf);
     hipFree(slots);
@@ -281,9 +280,8 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
         if (e == hipSuccess) e = hipMalloc(&ids[i], N * sizeof(uint32_t));
     }
     if (e == hipSuccess) e = hipMalloc(&keys, N * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&counts, C * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&buckets, C * sizeof(uint2));
     if (e == hipSuccess) e = hipMalloc(&rep, C * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&bucketStart, C * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&binStart, (B + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&binOf, N * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&slots, N * sizeof(uint32_t));
@@ -567,7 +565,7 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
     g.vel = vel[pc];
     g.ids = ids[ic];
     g.keys = keys;
-    g.counts = counts;
+    g.buckets = buckets;
     g.rep = rep;
     g.binStart = binStart;
     g.binOf = binOf;
@@ -579,7 +577,6 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
     g.velSorted = vel[1 - pc];
     g.idsSorted = ids[1 - ic];
     g.keysSorted = keysSorted;
-    g.bucketStart = bucketStart;
     g.restCount = restCount;
     g.rest = restList;
     g.binMax = binMax;
@@ -614,14 +611,14 @@ int Engine::run_substep(uint32_t j, bool prepareOnly, const sphdev::NeighbourSum
         // A prepare-only pass is followed by no force pass here: no neighbour list.
         const bool list = !prepareOnly;
         sphdev::DensityTiles tiles{binStart, restList, restCount, kRestBlocks};
-        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, bucketStart, counts, list ? nbrList : nullptr,
+        SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, buckets, list ? nbrList : nullptr,
                                        list ? nbrCount : nullptr, sums, stream,
                                        (restList && !sums) ? &tiles : nullptr));
     }
     if (prepareOnly) return SPH_OK;
     {
         Timer t(this, KC_FORCE);
-        SPH_HIP(sphdev::launch_force(L, g.posSorted, g.velSorted, bucketStart, counts, nbrList, nbrCount, pos[pc],
+        SPH_HIP(sphdev::launch_force(L, g.posSorted, g.velSorted, buckets, nbrList, nbrCount, pos[pc],
                                      vel[pc], force, stream));
     }
     ic = 1 - ic;
@@ -775,18 +772,19 @@ int Engine::read_grid(uint32_t scene, uint32_t* cellOf, uint32_t* cellCounts, ui
         SPH_HIP(hipStreamSynchronize(stream));
     }
     if (cellCounts || cellParticles) {
-        std::vector<uint32_t> cnt(C), starts(C), idv(n);
-        SPH_HIP(hipMemcpyAsync(cnt.data(), counts + s.cellOffset, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
-        SPH_HIP(hipMemcpyAsync(starts.data(), bucketStart + s.cellOffset, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
+        std::vector<uint2> bt(C);
+        std::vector<uint32_t> idv(n);
+        SPH_HIP(hipMemcpyAsync(bt.data(), buckets + s.cellOffset, (size_t)C * sizeof(uint2), hipMemcpyDeviceToHost,
+                               stream));
         if (n) SPH_HIP(hipMemcpyAsync(idv.data(), ids[ic] + s.offset, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
         SPH_HIP(hipStreamSynchronize(stream));
         for (uint32_t b = 0; b < C; ++b) {
-            const uint32_t c = cnt[b];
+            const uint32_t c = bt[b].y;
             if (cellCounts) cellCounts[b] = c;
             if (cellParticles) {
                 for (uint32_t k = 0; k < SPH_CELL_CAPACITY; ++k)
                     cellParticles[(size_t)b * SPH_CELL_CAPACITY + k] =
-                        k < c ? idv[starts[b] - s.offset + k] : 0xFFFFFFFFu;
+                        k < c ? idv[bt[b].x - s.offset + k] : 0xFFFFFFFFu;
             }
         }
     }
@@ -804,16 +802,15 @@ int Engine::read_sorted(uint32_t scene, uint32_t* keysOut, uint32_t* idsOut, uin
         SPH_HIP(hipMemcpyAsync(k.data(), keysSorted + s.offset, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
         if (idsOut) SPH_HIP(hipMemcpyAsync(idsOut, ids[ic] + s.offset, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
     }
-    std::vector<uint32_t> cnt(C);
-    if (bucketStarts) {
-        SPH_HIP(hipMemcpyAsync(cnt.data(), counts + s.cellOffset, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
-        SPH_HIP(hipMemcpyAsync(bucketStarts, bucketStart + s.cellOffset, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
-    }
+    std::vector<uint2> bt(bucketStarts ? C : 0);
+    if (bucketStarts)
+        SPH_HIP(hipMemcpyAsync(bt.data(), buckets + s.cellOffset, (size_t)C * sizeof(uint2), hipMemcpyDeviceToHost,
+                               stream));
     SPH_HIP(hipStreamSynchronize(stream));
     if (keysOut)
         for (uint32_t i = 0; i < n; ++i) keysOut[i] = k[i] - s.cellOffset;
     if (bucketStarts)
-        for (uint32_t b = 0; b < C; ++b) bucketStarts[b] = cnt[b] ? bucketStarts[b] - s.offset : 0xFFFFFFFFu;
+        for (uint32_t b = 0; b < C; ++b) bucketStarts[b] = bt[b].y ? bt[b].x - s.offset : 0xFFFFFFFFu;
     return SPH_OK;
 }
 

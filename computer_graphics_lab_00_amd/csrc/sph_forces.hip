@@ -23,14 +23,14 @@ __device__ inline uint32_t hash_cell(int32_t cx, int32_t cy, int32_t cz, uint32_
 
 __device__ inline int32_t cell_coord(float x, float h) { return (int32_t)floorf(x / h); }
 
-// Streaming (non-temporal) 16-byte accesses for the neighbour list, which is written once
-// and read once per substep and should not push particle state out of L2.
+// The neighbour list is written once and read once per substep. Its 16-byte groups are
+// written with ordinary stores, so the lines stay in the caches for the force pass that
+// reads them next (at 1M particles: force pass 0.15 ms against 0.17 ms when the groups were
+// written non-temporally, with the density pass unchanged); the force pass reads them as
+// streaming (non-temporal) loads, since nothing reads them again.
 typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void store_stream(uint4* p, uint4 v) {
-    u32x4_nt x = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(x, reinterpret_cast<u32x4_nt*>(p));
-}
+__device__ __forceinline__ void store_list(uint4* p, uint4 v) { *p = v; }
 
 __device__ __forceinline__ uint4 load_stream(const uint4* p) {
     const u32x4_nt x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_nt*>(p));
@@ -126,8 +126,8 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // launch keep the reference's arithmetic.
 template <bool SUMS, bool DEDUPE, bool FAST = false>
 __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) : 1)
-    k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
-              const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
+    k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint2* __restrict__ buckets,
+              uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
               float4* __restrict__ sums, float2* __restrict__ deviation) {
     const Where w = locate(L);
     const bool live = w.valid && w.d->active;
@@ -202,8 +202,9 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
 #pragma unroll
         for (uint32_t o = 0; o < 9; ++o) {
             bk[o] = hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz, k.cellMask);
-            cn[o] = counts[cellBase + bk[o]];
-            st[o] = bucketStart[cellBase + bk[o]];
+            const uint2 bd = buckets[cellBase + bk[o]];
+            st[o] = bd.x;
+            cn[o] = bd.y;
         }
 #pragma unroll
         for (uint32_t o = 0; o < 9; ++o) {
@@ -244,7 +245,7 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
                 const uint32_t done = listed < cap ? listed : cap;
                 for (; flushed + 4u <= done; flushed += 4u) {
                     const uint32_t r0 = flushed & 15u;
-                    store_stream(nbrList + (size_t)(flushed >> 2) * L.total + t,
+                    store_list(nbrList + (size_t)(flushed >> 2) * L.total + t,
                                  make_uint4(ring[r0][threadIdx.x], ring[r0 + 1u][threadIdx.x],
                                             ring[r0 + 2u][threadIdx.x], ring[r0 + 3u][threadIdx.x]));
                 }
@@ -278,7 +279,7 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
         if (listed <= cap && flushed < listed) {
             // the last, partial group
             const uint32_t r0 = flushed & 15u, r = listed - flushed;
-            store_stream(nbrList + (size_t)(flushed >> 2) * L.total + t,
+            store_list(nbrList + (size_t)(flushed >> 2) * L.total + t,
                          make_uint4(ring[r0][threadIdx.x], r > 1u ? ring[r0 + 1u][threadIdx.x] : 0u,
                                     r > 2u ? ring[r0 + 2u][threadIdx.x] : 0u, 0u));
         }
@@ -346,7 +347,7 @@ __device__ __forceinline__ void list_push(uint32_t (*ring)[kBlock], uint32_t m, 
     for (uint32_t g = 0; g < (B + 3) / 4; ++g) {
         if (flushed + 4u <= done) {
             const uint32_t r0 = flushed & (kRing - 1u);
-            store_stream(nbrList + (size_t)(flushed >> 2) * stride + t,
+            store_list(nbrList + (size_t)(flushed >> 2) * stride + t,
                          make_uint4(ring[r0][threadIdx.x], ring[r0 + 1u][threadIdx.x], ring[r0 + 2u][threadIdx.x],
                                     ring[r0 + 3u][threadIdx.x]));
             flushed += 4u;
@@ -366,7 +367,7 @@ __device__ __forceinline__ void density_finish(const StepConsts& k, float densit
     if (nbrCount) {
         if (listed <= cap && flushed < listed) {
             const uint32_t r0 = flushed & (kRing - 1u), r = listed - flushed;
-            store_stream(nbrList + (size_t)(flushed >> 2) * stride + t,
+            store_list(nbrList + (size_t)(flushed >> 2) * stride + t,
                          make_uint4(ring[r0][threadIdx.x], r > 1u ? ring[r0 + 1u][threadIdx.x] : 0u,
                                     r > 2u ? ring[r0 + 2u][threadIdx.x] : 0u, 0u));
         }
@@ -406,8 +407,8 @@ __device__ __forceinline__ uint32_t density_batch(const StepConsts& k, float px,
 
 // The plain walk of one listed particle (slot t of scene d), in global memory.
 __device__ __forceinline__ void density_walk_global(const SceneDesc* d, uint32_t t, float4* __restrict__ pos,
-                                                    float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
-                                                    const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList,
+                                                    float4* __restrict__ vel, const uint2* __restrict__ buckets,
+                                                    uint4* __restrict__ nbrList,
                                                     uint32_t* __restrict__ nbrCount, uint32_t (*ring)[kBlock],
                                                     size_t stride) {
     const StepConsts k = d->k;
@@ -425,8 +426,9 @@ __device__ __forceinline__ void density_walk_global(const SceneDesc* d, uint32_t
         for (uint32_t o = 0; o < 9; ++o) {
             const uint32_t b = cellBase + hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz,
                                                     k.cellMask);
-            cn[o] = counts[b];
-            st[o] = bucketStart[b];
+            const uint2 bd = buckets[b];
+            st[o] = bd.x;
+            cn[o] = bd.y;
         }
         for (uint32_t o = 0; o < 9; ++o) {
             uint32_t cnt = cn[o];
@@ -441,8 +443,8 @@ __device__ __forceinline__ void density_walk_global(const SceneDesc* d, uint32_t
 }
 
 __global__ void __launch_bounds__(kBlock, 8)
-    k_density_tiled(Launch L, const uint32_t* __restrict__ binStart, float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ bucketStart,
-                    const uint32_t* __restrict__ counts, uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
+    k_density_tiled(Launch L, const uint32_t* __restrict__ binStart, float4* __restrict__ pos, float4* __restrict__ vel, const uint2* __restrict__ buckets,
+                    uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
                     const uint2* __restrict__ rest, const uint32_t* __restrict__ restCount) {
     __shared__ float4 boxLo[kBoxCells];   // per neighbourhood cell: bucket box low corner, .w = first slot
     __shared__ float4 boxHi[kBoxCells];   // bucket box high corner, .w = member count (capped)
@@ -457,7 +459,7 @@ __global__ void __launch_bounds__(kBlock, 8)
             const uint2 r = rest[i];
             const SceneDesc* d = L.scenes + r.y;
             if (d->active)
-                density_walk_global(d, r.x, pos, vel, bucketStart, counts, nbrList, nbrCount, ring, L.total);
+                density_walk_global(d, r.x, pos, vel, buckets, nbrList, nbrCount, ring, L.total);
         }
         return;
     }
@@ -496,12 +498,13 @@ __global__ void __launch_bounds__(kBlock, 8)
     if (kc < nbox) {
         const int32_t cx = bx0 + (int32_t)(kc % BX), cy = by0 + (int32_t)((kc / BX) % BY), cz = bz0 + (int32_t)(kc / BXY);
         const uint32_t b = cellBase + hash_cell(cx, cy, cz, k.cellMask);
-        uint32_t cnt = counts[b];
+        const uint2 bd = buckets[b];
+        uint32_t cnt = bd.y;
         if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         uint32_t first = 0;
         if (cnt) {
-            first = bucketStart[b];
+            first = bd.x;
             for (uint32_t m = 0; m < cnt; m += 4) {
                 float q[4][3];
 #pragma unroll
@@ -812,8 +815,7 @@ __device__ __forceinline__ void pair_force(const StepConsts& k, bool fast, const
 template <bool FAST>
 __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
     k_force(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
-            const uint32_t* __restrict__ bucketStart, const uint32_t* __restrict__ counts,
-            const uint4* __restrict__ nbrList, const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut,
+            const uint2* __restrict__ buckets, const uint4* __restrict__ nbrList, const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut,
             float4* __restrict__ velOut, float4* __restrict__ forceOut) {
     const Where w = locate(L);
     if (!w.valid) return;
@@ -885,9 +887,10 @@ __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
                 for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
                     const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
                     if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
-                    uint32_t cnt = counts[cellBase + b];
+                    const uint2 bd = buckets[cellBase + b];
+                    uint32_t cnt = bd.y;
                     if (cnt == 0) continue;
-                    const uint32_t lo = bucketStart[cellBase + b];
+                    const uint32_t lo = bd.x;
                     if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
                     for (uint32_t j = 0; j < cnt; ++j) {
                         const uint32_t u = lo + j;
@@ -949,49 +952,49 @@ __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
 
 }  // namespace
 
-hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint32_t* bucketStart,
-                          const uint32_t* counts, uint4* nbrList, uint32_t* nbrCount,
+hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2* buckets, uint4* nbrList,
+                          uint32_t* nbrCount,
                           const NeighbourSums* sums, hipStream_t s, const DensityTiles* tiles) {
     if (L.blocks == 0) return hipSuccess;
     const bool dedupe = (L.modes & SPH_MODE_DEDUPE) != 0;
     if (sums) {
         if (dedupe)
-            k_density<true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nullptr, nullptr,
+            k_density<true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nullptr, nullptr,
                                                               sums->sums, sums->deviation);
         else
-            k_density<true, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nullptr, nullptr,
+            k_density<true, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nullptr, nullptr,
                                                                sums->sums, sums->deviation);
     } else if (tiles && !dedupe && !(L.modes & SPH_MODE_FAST_MATH) && L.tiles > 0 && L.tileScene) {
         k_density_tiled<<<L.tiles + tiles->restBlocks, kBlock, 0, s>>>(L, tiles->binStart, pos, vel,
-                                                                       bucketStart, counts, nbrList, nbrCount,
+                                                                       buckets, nbrList, nbrCount,
                                                                        tiles->rest, tiles->restCount);
     } else {
         const bool fast = (L.modes & SPH_MODE_FAST_MATH) != 0;
         if (dedupe && fast)
-            k_density<false, true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList,
+            k_density<false, true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList,
                                                                      nbrCount, nullptr, nullptr);
         else if (dedupe)
-            k_density<false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
+            k_density<false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount,
                                                                nullptr, nullptr);
         else if (fast)
-            k_density<false, false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList,
+            k_density<false, false, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList,
                                                                       nbrCount, nullptr, nullptr);
         else
-            k_density<false, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount,
+            k_density<false, false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount,
                                                                 nullptr, nullptr);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
-                        const uint32_t* counts, const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
+hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, const uint2* buckets,
+                        const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
     if (L.modes & SPH_MODE_FAST_MATH)
-        k_force<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut,
+        k_force<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount, posOut, velOut,
                                                   forceOut);
     else
-        k_force<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, nbrList, nbrCount, posOut, velOut,
+        k_force<false><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount, posOut, velOut,
                                                    forceOut);
     return hipGetLastError();
 }

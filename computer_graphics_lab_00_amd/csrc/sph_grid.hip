@@ -13,8 +13,8 @@
 // sit next to each other in HBM, which is what the 27-bucket gathers read.
 // All scenes of a batch go through each step in one launch (workgroups find their scene
 // in a table); the single solver is a batch of one.
-//   1. clear:   counts = 0, rep = ~0, bin counts = 0
-//   2. keys:    key = spatialHash(spatialCell(p, h)); counts[key] += 1; rep[key] min= bin
+//   1. clear:   buckets = (0, 0), rep = ~0, bin counts = 0
+//   2. keys:    key = spatialHash(spatialCell(p, h)); buckets[key].count += 1; rep[key] min= bin
 //               (wave-level run aggregation: the state is already spatially sorted, so
 //               neighbouring lanes mostly share a bucket); each scene's filler bin = its
 //               unused capacity, so one global scan aligns every scene to its slots
@@ -24,7 +24,7 @@
 //   5. scatter: (key, id, source) lands in its bin's range
 //   6. rank:    final slot = bin start + rank of (key, id) among the bin's members; the
 //               particle's pos / vel / id are copied there, and the first member of
-//               each bucket records bucketStart[key]
+//               each bucket records its slot as buckets[key].first
 // The atomics only ever produce counts and provisional slots; every final position is
 // a pure function of the keys and indices, so the result is deterministic.
 #include "sph_device.h"
@@ -74,14 +74,14 @@ __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 // Bit mask of lanes at or below this lane.
 __device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ull : ((2ull << lane) - 1ull); }
 
-// counts = 0 and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
+// buckets = (0, 0) and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
 __global__ void __launch_bounds__(kBlock)
-    k_grid_clear(uint32_t* __restrict__ counts, uint32_t* __restrict__ rep, uint32_t cells,
+    k_grid_clear(uint2* __restrict__ buckets, uint32_t* __restrict__ rep, uint32_t cells,
                  uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount) {
     const uint32_t stride = gridDim.x * kBlock;
     if (restCount && blockIdx.x == 0 && threadIdx.x == 0) *restCount = 0u;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride) {
-        counts[i] = 0u;
+        buckets[i] = make_uint2(0u, 0u);
         rep[i] = 0xFFFFFFFFu;
     }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < bins; i += stride) binCount[i] = 0u;
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ ids, uin
 // Also sets each scene's filler bin (workgroup 0; the bin counts were cleared before and
 // nothing else touches them until k_bin_slots).
 __global__ void __launch_bounds__(kBlock)
-    k_keys(Launch L, const float4* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ counts,
+    k_keys(Launch L, const float4* __restrict__ pos, uint32_t* __restrict__ keys, uint2* __restrict__ buckets,
            uint32_t* __restrict__ rep, uint32_t* __restrict__ binCount) {
     if (blockIdx.x == 0) {
         for (uint32_t s = threadIdx.x; s < L.numScenes; s += kBlock) {
@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(kBlock)
     if (head) {
         const uint64_t above = heads & ~lanes_upto(lane);
         const uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : (uint32_t)__popcll(live);
-        atomicAdd(&counts[key], end - lane);
+        atomicAdd(&buckets[key].y, end - lane);
     }
     if (w.valid && (head || prevBin != bin)) atomicMin(&rep[key], bin);
 }
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kBlock)
                    const uint32_t* __restrict__ binOf, const uint32_t* __restrict__ binStart,
                    const float4* __restrict__ pos, const float4* __restrict__ vel, float4* __restrict__ posSorted,
                    float4* __restrict__ velSorted, uint32_t* __restrict__ idsSorted,
-                   uint32_t* __restrict__ keysSorted, uint32_t* __restrict__ bucketStart, uint2* __restrict__ rest,
+                   uint32_t* __restrict__ keysSorted, uint2* __restrict__ buckets, uint2* __restrict__ rest,
                    uint32_t* __restrict__ restCount, const uint32_t* __restrict__ binMax,
                    uint32_t* __restrict__ binMaxHost) {
     // Workgroup 0 reduces k_bin_slots's per-workgroup maxima (final: that launch has
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kBlock)
         velSorted[dst] = vel[src];
         idsSorted[dst] = me.y;
         keysSorted[dst] = me.x;
-        if (first) bucketStart[me.x] = dst;
+        if (first) buckets[me.x].x = dst;
     }
     if (!rest) return;   // the same for every lane
     // Is the particle's own cell inside the half-tile its bin belongs to? (bins are cells when
@@ -421,7 +421,7 @@ hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t 
     const uint32_t span = L.totalCells > L.totalBins ? L.totalCells : L.totalBins;
     uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
-    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.counts, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount);
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.buckets, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount);
     return hipGetLastError();
 }
 
@@ -430,7 +430,7 @@ hipError_t launch_keys(const Launch& L, const GridBuffers& b, hipStream_t s) {
         k_grid_fillers<<<1, kBlock, 0, s>>>(L, b.binStart);
         return hipGetLastError();
     }
-    k_keys<<<L.blocks, kBlock, 0, s>>>(L, b.pos, b.keys, b.counts, b.rep, b.binStart);
+    k_keys<<<L.blocks, kBlock, 0, s>>>(L, b.pos, b.keys, b.buckets, b.rep, b.binStart);
     return hipGetLastError();
 }
 
@@ -454,7 +454,7 @@ hipError_t launch_scatter(const Launch& L, const GridBuffers& b, hipStream_t s) 
 hipError_t launch_rank_reorder(const Launch& L, const GridBuffers& b, hipStream_t s) {
     const bool tiles = L.tiles > 0 && b.rest && b.restCount;
     k_rank_reorder<<<L.blocks, kBlock, 0, s>>>(L, b.pairKI, b.pairSrc, b.binOf, b.binStart, b.pos, b.vel,
-                                              b.posSorted, b.velSorted, b.idsSorted, b.keysSorted, b.bucketStart,
+                                              b.posSorted, b.velSorted, b.idsSorted, b.keysSorted, b.buckets,
                                               tiles ? b.rest : nullptr, b.restCount, b.binMax,
                                               b.binMax ? b.binMaxHost : nullptr);
     return hipGetLastError();

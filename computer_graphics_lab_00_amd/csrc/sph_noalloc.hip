@@ -45,10 +45,9 @@ constexpr size_t kOffSlots = kOffBinOf + up256(kMaxN * 4ull);
 constexpr size_t kOffKeysSorted = kOffSlots + up256(kMaxN * 4ull);
 constexpr size_t kOffPairSrc = kOffKeysSorted + up256(kMaxN * 4ull);
 constexpr size_t kOffPairKI = kOffPairSrc + up256(kMaxN * 4ull);
-constexpr size_t kOffCounts = kOffPairKI + up256(kMaxN * 8ull);
-constexpr size_t kOffRep = kOffCounts + up256(kMaxC * 4ull);
-constexpr size_t kOffBucketStart = kOffRep + up256(kMaxC * 4ull);
-constexpr size_t kOffBinStart = kOffBucketStart + up256(kMaxC * 4ull);
+constexpr size_t kOffBuckets = kOffPairKI + up256(kMaxN * 8ull);
+constexpr size_t kOffRep = kOffBuckets + up256(kMaxC * 8ull);
+constexpr size_t kOffBinStart = kOffRep + up256(kMaxC * 4ull);
 constexpr size_t kOffBlockSums = kOffBinStart + up256((kMaxBins + 2) * 4ull);
 constexpr size_t kOffPosSorted = kOffBlockSums + up256((kMaxScanTiles + 1) * 4ull);
 constexpr size_t kOffVelSorted = kOffPosSorted + up256((kMaxN + sphdev::kPosPad) * 16ull);
@@ -178,7 +177,7 @@ extern "C" int sph_substep_device(const SphParams* p, float dt, uint32_t modeFla
     g.vel = reinterpret_cast<const float4*>(velIn);
     g.ids = ids;
     g.keys = reinterpret_cast<uint32_t*>(base + kOffKeys);
-    g.counts = reinterpret_cast<uint32_t*>(base + kOffCounts);
+    g.buckets = reinterpret_cast<uint2*>(base + kOffBuckets);
     g.rep = reinterpret_cast<uint32_t*>(base + kOffRep);
     g.binStart = reinterpret_cast<uint32_t*>(base + kOffBinStart);
     g.binOf = reinterpret_cast<uint32_t*>(base + kOffBinOf);
@@ -190,7 +189,6 @@ extern "C" int sph_substep_device(const SphParams* p, float dt, uint32_t modeFla
     g.velSorted = reinterpret_cast<float4*>(base + kOffVelSorted);
     g.idsSorted = reinterpret_cast<uint32_t*>(base + kOffIdsSorted);
     g.keysSorted = reinterpret_cast<uint32_t*>(base + kOffKeysSorted);
-    g.bucketStart = reinterpret_cast<uint32_t*>(base + kOffBucketStart);
     float4* posNext = reinterpret_cast<float4*>(base + kOffPosNext);
     float4* velNext = reinterpret_cast<float4*>(base + kOffVelNext);
 
@@ -204,9 +202,8 @@ extern "C" int sph_substep_device(const SphParams* p, float dt, uint32_t modeFla
     SPH_HIP(sphdev::launch_scatter(L, g, s));
     SPH_HIP(sphdev::launch_rank_reorder(L, g, s));
     // density + EOS in place, then force + integrate + boundaries into the next state
-    SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, g.bucketStart, g.counts, nullptr, nullptr, nullptr,
-                                   s));
-    SPH_HIP(sphdev::launch_force(L, g.posSorted, g.velSorted, g.bucketStart, g.counts, nullptr, nullptr, posNext,
+    SPH_HIP(sphdev::launch_density(L, g.posSorted, g.velSorted, g.buckets, nullptr, nullptr, nullptr, s));
+    SPH_HIP(sphdev::launch_force(L, g.posSorted, g.velSorted, g.buckets, nullptr, nullptr, posNext,
                                  velNext, nullptr, s));
     // back to index order, into the caller's buffers
     SPH_HIP(sphdev::launch_unpermute(posNext, g.idsSorted, reinterpret_cast<float4*>(posOut), count, s));

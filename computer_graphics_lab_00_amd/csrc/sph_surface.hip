@@ -59,8 +59,8 @@ __device__ inline int32_t field_cell(float origin, float cell, uint32_t i, float
 // the sorted global arrays: the 27 buckets around the point's cell in the reference's order
 // (z, y, x), a bucket two neighbour cells fold into summed once (:319-326), at most 32
 // particles per bucket.
-__device__ float sample_global(const float4* __restrict__ pos, const uint32_t* __restrict__ bucketStart,
-                               const uint32_t* __restrict__ counts, const FieldArgs& a, float px, float py, float pz,
+__device__ float sample_global(const float4* __restrict__ pos, const uint2* __restrict__ buckets,
+                               const FieldArgs& a, float px, float py, float pz,
                                int32_t bx, int32_t by, int32_t bz) {
     uint32_t bk[27];
 #pragma unroll
@@ -72,10 +72,11 @@ __device__ float sample_global(const float4* __restrict__ pos, const uint32_t* _
 #pragma unroll
         for (int q = 0; q < o; ++q) dup |= bk[q] == bk[o];
         if (dup) continue;
-        uint32_t cnt = counts[bk[o]];
+        const uint2 bd = buckets[bk[o]];
+        uint32_t cnt = bd.y;
         if (cnt == 0) continue;
         if (cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
-        const uint32_t lo = bucketStart[bk[o]];
+        const uint32_t lo = bd.x;
         for (uint32_t k = 0; k < cnt; ++k) {
             const float4 q = pos[lo + k];
             const float ex = q.x - px, ey = q.y - py, ez = q.z - pz;
@@ -104,8 +105,8 @@ static_assert(kTileX * kTileY * kTileZ == kThreads, "one sample per thread");
 static_assert(kBoxCells <= 4 * kThreads, "each thread scans at most four cells");
 
 __global__ void __launch_bounds__(kThreads)
-    k_field(const float4* __restrict__ pos, const uint32_t* __restrict__ bucketStart,
-            const uint32_t* __restrict__ counts, FieldArgs a, uint32_t tilesX, uint32_t tilesY,
+    k_field(const float4* __restrict__ pos, const uint2* __restrict__ buckets, FieldArgs a, uint32_t tilesX,
+            uint32_t tilesY,
             float* __restrict__ field) {
     __shared__ float4 sp[kLdsParticles + kBatch];   // a batch may read past the last cell
     __shared__ uint32_t cellOff[kBoxCells + 1];
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(kThreads)
                 const int32_t cy = box[1] + (int32_t)((c / (uint32_t)box[3]) % (uint32_t)box[4]);
                 const int32_t cz = box[2] + (int32_t)(c / ((uint32_t)box[3] * (uint32_t)box[4]));
                 const uint32_t bk = hash_cell(cx, cy, cz, a.cellMask);
-                uint32_t cnt = counts[bk];
+                uint32_t cnt = buckets[bk].y;
                 if (cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
                 cellBk[c] = bk;
                 cellOff[c] = cnt;
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(kThreads)
             const uint32_t lane = tid & 63u, wave = tid >> 6;
             for (uint32_t c = wave; c < ncell; c += kThreads / 64u) {
                 const uint32_t n = cellOff[c + 1] - cellOff[c];
-                if (lane < n) sp[cellOff[c] + lane] = pos[bucketStart[cellBk[c]] + lane];
+                if (lane < n) sp[cellOff[c] + lane] = pos[buckets[cellBk[c]].x + lane];
             }
         }
         __syncthreads();
@@ -200,7 +201,7 @@ __global__ void __launch_bounds__(kThreads)
     const int32_t bx = (int32_t)floorf(px / a.radius), by = (int32_t)floorf(py / a.radius),
                   bz = (int32_t)floorf(pz / a.radius);
     if (!useLds) {
-        field[i] = sample_global(pos, bucketStart, counts, a, px, py, pz, bx, by, bz);
+        field[i] = sample_global(pos, buckets, a, px, py, pz, bx, by, bz);
         return;
     }
     // the 27 cells as box indices; the box holds every one of them
@@ -458,8 +459,8 @@ int field_on_device(sphhost::Engine& e, const SphParams& p, uint32_t resolution,
     a.cellMask = p.cellCount - 1u;
     const uint32_t tilesX = (s + kTileX - 1) / kTileX, tilesY = (s + kTileY - 1) / kTileY,
                    tilesZ = (s + kTileZ - 1) / kTileZ;
-    k_field<<<tilesX * tilesY * tilesZ, kThreads, 0, e.stream>>>(e.sorted_pos(), e.bucket_starts(), e.bucket_counts(),
-                                                                 a, tilesX, tilesY, field);
+    k_field<<<tilesX * tilesY * tilesZ, kThreads, 0, e.stream>>>(e.sorted_pos(), e.bucket_table(), a, tilesX,
+                                                                 tilesY, field);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }

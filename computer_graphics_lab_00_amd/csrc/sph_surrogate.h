@@ -74,9 +74,8 @@ constexpr uint32_t kKnn = 8;
 //   6k .. 6k+2 : (x_i - x_j) / h
 //   6k+3 .. 6k+5 : (v_j - v_i) / vScale
 // (zeros for missing neighbours), then 48: 100 (rho / rho0 - 1), 49: h / kLengthScale.
-hipError_t launch_knn_features(const sphdev::Launch& L, const float4* pos, const float4* vel,
-                               const uint32_t* bucketStart, const uint32_t* counts, const uint32_t* ids,
-                               float vScale, uint16_t* x, hipStream_t s);
+hipError_t launch_knn_features(const sphdev::Launch& L, const float4* pos, const float4* vel, const uint2* buckets,
+                               const uint32_t* ids, float vScale, uint16_t* x, hipStream_t s);
 hipError_t launch_integrate_macro(const sphdev::Launch& L, const float4* pos, const float4* vel, const float* y,
                                   uint32_t ldY, float aScale, uint32_t substeps, const float4* sums,
                                   float smoothing, float4* posOut, float4* velOut, float4* posKeep,

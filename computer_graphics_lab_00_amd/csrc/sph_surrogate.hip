@@ -577,7 +577,7 @@ extern "C" int sph_dataset_record(SphDataset* d, SphSolver* solver, const SphPar
     if (int st = e.prepare(dtSub, &ns)) return st;
     const sphdev::Launch L = e.plan_launch();
     if (d->featureKind == sphsur::kFeaturesKnn8)
-        SPH_HIP(sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_starts(), e.bucket_counts(),
+        SPH_HIP(sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_table(),
                                             e.sorted_ids(), sphsur::kVelocityScale, d->xAll, s));
     else
         SPH_HIP(sphsur::launch_features(L, e.sorted_pos(), d->sums, sphsur::kVelocityScale, d->xAll, s));
@@ -752,7 +752,7 @@ int predict(SphNet* n, sphhost::Engine& e, uint32_t slot) {
     if (timed) SPH_HIP(hipEventRecord(n->ev[2 * slot], s));
     if (n->featureKind == sphsur::kFeaturesKnn8) {
         // feature rows of the 8 nearest neighbours, then the network on the rows
-        SPH_HIP(sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_starts(), e.bucket_counts(),
+        SPH_HIP(sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_table(),
                                             e.sorted_ids(), sphsur::kVelocityScale, n->ix, s));
         SPH_HIP(sphmlp::launch_mlp_fused(n->ix, e.sc[0].n, n->wb, n->w, reinterpret_cast<float4*>(n->iy), 0, s));
     } else {
@@ -824,8 +824,8 @@ extern "C" int sph_knn_features(SphSolver* solver, const SphParams* p, float dtS
     int st = e.prepare(dtSub, nullptr);
     if (!st) {
         const sphdev::Launch L = e.plan_launch();
-        hipError_t he = sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_starts(),
-                                                    e.bucket_counts(), e.sorted_ids(), sphsur::kVelocityScale, all,
+        hipError_t he = sphsur::launch_knn_features(L, e.sorted_pos(), e.sorted_vel(), e.bucket_table(),
+                                                    e.sorted_ids(), sphsur::kVelocityScale, all,
                                                     e.stream);
         if (he == hipSuccess) he = sphsur::launch_unpermute_rows(all, e.sorted_ids(), n, ordered, e.stream);
         if (he == hipSuccess) he = hipMemcpyAsync(x, ordered, (size_t)n * 64 * 2, hipMemcpyDeviceToHost, e.stream);

@@ -64,8 +64,8 @@ __device__ __forceinline__ bool knn_before(float r, uint32_t id, float r2, uint3
 // 8-nearest-neighbour feature rows (launch_knn_features in sph_surrogate.h), slot order.
 __global__ void __launch_bounds__(kBlock)
     k_knn_features(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                   const uint32_t* __restrict__ bucketStart, const uint32_t* __restrict__ counts,
-                   const uint32_t* __restrict__ ids, float invV, uint16_t* __restrict__ x) {
+                   const uint2* __restrict__ buckets, const uint32_t* __restrict__ ids, float invV,
+                   uint16_t* __restrict__ x) {
     const Where w = locate(L);
     if (!w.valid) return;
     const StepConsts k = w.d->k;
@@ -87,10 +87,11 @@ __global__ void __launch_bounds__(kBlock)
         for (int32_t dy = -1; dy <= 1; ++dy)
             for (int32_t dx = -1; dx <= 1; ++dx) {
                 const uint32_t b = cellBase + knn_hash(cx + dx, cy + dy, cz + dz, k.cellMask);
-                uint32_t cnt = counts[b];
+                const uint2 bd = buckets[b];
+                uint32_t cnt = bd.y;
                 if (cnt == 0) continue;
                 if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
-                const uint32_t lo = bucketStart[b];
+                const uint32_t lo = bd.x;
                 for (uint32_t j = 0; j < cnt; ++j) {
                     const uint32_t u = lo + j;
                     if (u == t) continue;
@@ -384,10 +385,10 @@ hipError_t launch_features(const Launch& L, const float4* pos, const float4* sum
     return hipGetLastError();
 }
 
-hipError_t launch_knn_features(const Launch& L, const float4* pos, const float4* vel, const uint32_t* bucketStart,
-                               const uint32_t* counts, const uint32_t* ids, float vScale, uint16_t* x, hipStream_t s) {
+hipError_t launch_knn_features(const Launch& L, const float4* pos, const float4* vel, const uint2* buckets,
+                               const uint32_t* ids, float vScale, uint16_t* x, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
-    k_knn_features<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, bucketStart, counts, ids, 1.0f / vScale, x);
+    k_knn_features<<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, ids, 1.0f / vScale, x);
     return hipGetLastError();
 }
 

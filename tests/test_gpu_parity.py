@@ -154,6 +154,24 @@ def test_s16k_one_substep(cells, flags, oracle_lib):
     gpu.close()
 
 
+@pytest.mark.parametrize("per_cell", [2.25, 3.0])
+def test_tiled_density_dense_neighbourhoods(per_cell, oracle_lib):
+    """The tiled density pass on lattices denser than the dam breaks': spacing h / 2.25
+    (about 11 particles per cell) and h / 3 (about 27 per cell), so buckets and
+    neighbour lists are two to three times as long. One substep is bit-exact."""
+    say(f"case: tiled density pass, lattice spacing h / {per_cell}")
+    h = 0.04
+    w = fluid.Workload(f"dense_{per_cell}", (32, 32, 16), h / per_cell, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h,
+                       cell_count=32768)
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    rng = np.random.default_rng(3)
+    vel[:, :3] = rng.normal(0, 0.5, (w.count, 3)).astype(np.float32)
+    gpu, ora = one_substep_both(p, pos, vel, 1.0 / 3000.0, oracle_lib, flags=capi.CFG_TILED_DENSITY)
+    compare_substep(gpu, ora, p, f"tiled, spacing h/{per_cell}")
+    gpu.close()
+
+
 def test_neighbour_list_overflow_falls_back(oracle_lib):
     """cellCount = 1 and 40 particles: every particle has 27 x 32 in-range entries, far
     past the list capacity, so the force pass walks the buckets; still bit-exact."""
@@ -423,12 +441,12 @@ def test_m1_clamped_cellcount_one_substep(oracle_lib):
     gpu.close()
 
 
-@pytest.mark.parametrize("cells", [1 << 21, 65536])
-def test_m1_developed_state_one_substep(cells, oracle_lib):
+@pytest.mark.parametrize("cells,flags", [(1 << 21, 0), (65536, 0), (1 << 21, capi.CFG_TILED_DENSITY)])
+def test_m1_developed_state_one_substep(cells, flags, oracle_lib):
     """One substep bit-exact from the 1M dam break after 100 steps (800 substeps) of the
     HIP path itself: a disordered, compressed state where buckets run past 32 members
     and the collapse has piled particles against the floor and walls."""
-    say(f"case: m1 developed state (100 steps), cellCount {cells}")
+    say(f"case: m1 developed state (100 steps), cellCount {cells}, flags {flags}")
     w = fluid.m1(cells)
     p = w.params()
     pos, vel = w.initial_state(p)
@@ -438,7 +456,7 @@ def test_m1_developed_state_one_substep(cells, oracle_lib):
             run.step(p, 1.0 / 600.0)
         pos, vel = run.state()
     dt = 1.0 / 600.0 / 8
-    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib, flags=flags)
     counts = ora.grid(cells)[1]
     say(f"  buckets over 32 members: {(counts > 32).sum()}, largest {counts.max()}, "
         f"density range {pos[:, 3].min():.1f}..{pos[:, 3].max():.1f}")
