@@ -35,6 +35,7 @@ SOURCES = [
     "sph_surface.hip",
     "sph_output.hip",
     "sph_noalloc.hip",
+    "sph_arith_check.hip",
 ]
 
 COMMON_FLAGS = [

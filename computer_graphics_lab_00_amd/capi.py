@@ -281,6 +281,8 @@ def _declare(lib):
         "sph_dataset_set_features": (C.c_int, [P, C.c_uint32]),
         "sph_knn_features": (C.c_int, [P, C.POINTER(SphParams), C.c_float, _HP]),
         "sph_density_deviation": (C.c_int, [P, P, C.POINTER(SphParams), C.c_float, _DP]),
+        "sph_check_exact_arithmetic": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "sph_short_division": (C.c_int, [P, _UP]),
         "sph_substep_device": (C.c_int, [C.POINTER(SphParams), C.c_float, C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     }

@@ -99,6 +99,12 @@ extern "C" int sph_max_bin_occupancy(SphSolver* s, uint32_t* out) {
     return SPH_OK;
 }
 
+extern "C" int sph_short_division(SphSolver* s, uint32_t* out) {
+    sphhost::clear_error();
+    if (!s || !out) return fail(SPH_ERR_INVALID_ARGUMENT, "sph_short_division: null argument");
+    return s->e.short_division(0, out);
+}
+
 extern "C" uint32_t sph_particle_count(const SphSolver* s) { return s ? s->e.sc[0].n : 0u; }
 
 extern "C" int sph_read_particles(SphSolver* s, float* pos4, float* vel4) {

@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include "sph_c_api.h"
+#include "sph_exact.h"
 
 namespace sphdev {
 
@@ -62,6 +63,7 @@ struct StepConsts {
     uint32_t colliderCount;
     uint32_t fieldCount;
     uint32_t listCap;       // neighbour-list slots per particle (0: no list)
+    uint32_t shortDivide;   // 1: h, mass and viscosity * mass admit the force pass's short division
 };
 
 // Spatial bins that order a scene's buckets in memory: the container's cells (plus a
@@ -108,7 +110,42 @@ struct Launch {
     uint32_t modes;               // OR of the scenes' SPH_MODE_* flags (picks kernel variants)
     const uint16_t* tileScene = nullptr;   // scene of each half-tile workgroup (32 spatial bins)
     uint32_t tiles = 0;                    // half-tile workgroups over all scenes (0: none planned)
+    // When set: zeroed by the grid clear, made nonzero by the reorder and the density pass when
+    // some particle's state lies outside the ranges of the force pass's short division
+    // (coord_in_range and the predicates beside it); the force pass then divides with the
+    // IEEE sequences.
+    uint32_t* rangeGuard = nullptr;
 };
+
+// Particle state inside which every operand of the force pass's short division and square
+// root (sph_exact.h) lies in the admitted ranges, for a scene whose shortDivide is set (h at
+// most 2^26, mass in [2^-60, 2^15], viscosity * mass 0 or in [2^-40, 2^8]):
+//  - a nonzero coordinate has magnitude at least 2^-67, so a nonzero coordinate difference
+//    (the numerators x_i - x_j; the pair's distance is below h) has magnitude at least 2^-90;
+//  - a nonzero velocity component has magnitude in [2^-33, 2^30], so a nonzero velocity
+//    difference lies in [2^-56, 2^31] and times viscosity * mass in [2^-96, 2^39];
+//  - pressure is 0 or in [2^-30, 2^30], so -mass (p_i + p_j) is 0 or in [2^-90, 2^46], and
+//    its quotient by the density and that quotient's half stay normal;
+//  - density (clamped below at 1e-6 > 2^-20) is at most 2^26, as is the distance (below h).
+// NaN and infinities fail every test.
+__device__ __forceinline__ bool coord_in_range(float c) {
+    const float a = fabsf(c);
+    return c == 0.0f || (a >= 0x1p-67f && a <= 0x1p60f);
+}
+__device__ __forceinline__ bool velocity_in_range(float v) {
+    const float a = fabsf(v);
+    return v == 0.0f || (a >= 0x1p-33f && a <= 0x1p30f);
+}
+__device__ __forceinline__ bool pressure_in_range(float p) { return p == 0.0f || (p >= 0x1p-30f && p <= 0x1p30f); }
+__device__ __forceinline__ bool density_in_range(float rho) { return rho <= 0x1p26f; }
+
+// Raise the guard once per wave that has a lane out of range (bad is rare; the first such
+// lane writes).
+__device__ __forceinline__ void raise_range_guard(uint32_t* guard, bool bad) {
+    if (!guard) return;
+    const uint64_t m = __ballot(bad);
+    if (m && (threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)m) - 1)) atomicOr(guard, 1u);
+}
 
 struct GridBuffers {
     const float4* pos;      // state before the sort (previous layout)

@@ -67,6 +67,8 @@ public:
     int read_grid(uint32_t scene, uint32_t* cellOf, uint32_t* cellCounts, uint32_t* cellParticles);
     int read_sorted(uint32_t scene, uint32_t* keys, uint32_t* ids, uint32_t* bucketStarts);
     int read_forces(uint32_t scene, float* force4);
+    // 1 if the last substep of `scene` ran the force pass's short division, else 0.
+    int short_division(uint32_t scene, uint32_t* out);
 
     int profile_read(const char** names, double* ms, uint32_t* launches, uint32_t capacity);
 
@@ -149,6 +151,7 @@ private:
     uint32_t* binMax = nullptr;         // per bin-slot workgroup: its largest bin occupancy
     uint32_t* binMaxHost = nullptr;     // its copy in mapped host memory (host address)
     uint32_t* binMaxHostDev = nullptr;  // the same memory's device address
+    uint32_t* rangeGuard = nullptr;     // nonzero: some particle's state is outside the short division's ranges
     void* hostStage = nullptr;
     size_t hostStageBytes = 0;
     std::vector<uint8_t> planKey;

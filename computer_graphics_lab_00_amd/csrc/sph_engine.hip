@@ -99,6 +99,13 @@ sphdev::StepConsts make_consts(const SceneHost& s, const SphParams& p, float dt,
     // The force pass may reuse the density pass's in-range list only if its own cut-off
     // (r2 < r2Cut) selects a subset of the density's (r2 < h2).
     k.listCap = (hasList && s.list && k.r2Cut <= k.h2) ? 64u : 0u;
+    // The force pass's short division is exact for these constants (sph_exact.h, and the
+    // particle-state ranges of coord_in_range and the predicates beside it in sph_device.h).
+    const float vm = k.viscMass;
+    k.shortDivide = (h > 0.0f && h <= 0x1p26f && k.mass >= 0x1p-60f && k.mass <= 0x1p15f &&
+                     (vm == 0.0f || (vm >= 0x1p-40f && vm <= 0x1p8f)))
+                        ? 1u
+                        : 0u;
     return k;
 }
 
@@ -211,6 +218,8 @@ void Engine::free_all() {
     hipFree(restCount);
     hipFree(binMax);
     binMax = nullptr;
+    hipFree(rangeGuard);
+    rangeGuard = nullptr;
     if (binMaxHost) hipHostFree(binMaxHost);
     binMaxHost = binMaxHostDev = nullptr;
     if (hostStage) hipHostFree(hostStage);
@@ -310,6 +319,8 @@ int Engine::create(const std::vector<std::pair<uint32_t, uint32_t>>& caps, int d
                      (maxBlocks + 1) * sizeof(uint16_t) + ((size_t)maxTiles + 1) * sizeof(uint16_t) + 512;
     if (e == hipSuccess) e = hipHostMalloc(&hostStage, hostStageBytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&binMax, (maxBlocks + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&rangeGuard, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(rangeGuard, 0xFF, sizeof(uint32_t));   // set by every grid build
     if (e == hipSuccess) e = hipHostMalloc(&binMaxHost, sizeof(uint32_t), hipHostMallocMapped);
     if (e == hipSuccess) {
         *binMaxHost = 0u;
@@ -420,6 +431,7 @@ sphdev::Launch Engine::launch_for(uint32_t j) const {
     for (const SceneHost& s : sc) L.modes |= s.mode;
     L.tileScene = dTileScene;
     L.tiles = dTileScene ? planTiles : 0u;
+    L.rangeGuard = rangeGuard;
     return L;
 }
 
@@ -811,6 +823,18 @@ int Engine::read_sorted(uint32_t scene, uint32_t* keysOut, uint32_t* idsOut, uin
         for (uint32_t i = 0; i < n; ++i) keysOut[i] = k[i] - s.cellOffset;
     if (bucketStarts)
         for (uint32_t b = 0; b < C; ++b) bucketStarts[b] = bt[b].y ? bt[b].x - s.offset : 0xFFFFFFFFu;
+    return SPH_OK;
+}
+
+int Engine::short_division(uint32_t scene, uint32_t* out) {
+    if (scene >= sc.size()) return fail(SPH_ERR_INVALID_ARGUMENT, "scene index out of range");
+    if (int st = set_device()) return st;
+    uint32_t guard = 1u;
+    SPH_HIP(hipStreamSynchronize(stream));
+    SPH_HIP(hipMemcpy(&guard, rangeGuard, sizeof(guard), hipMemcpyDeviceToHost));
+    const SceneHost& s = sc[scene];
+    const sphdev::StepConsts k = make_consts(s, s.params, 1.0f, false);
+    *out = hasGrid && s.n && guard == 0u && k.shortDivide ? 1u : 0u;
     return SPH_OK;
 }
 

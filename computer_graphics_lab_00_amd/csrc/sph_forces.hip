@@ -11,6 +11,8 @@
 // Consecutive lanes hold particles of the same bucket (the sort put them together), so
 // a wave's candidate loads hit the same few cache lines: neighbour reads are served
 // from L1/L2 and HBM sees each particle's state about once per pass.
+#include <type_traits>
+
 #include "sph_device.h"
 
 namespace sphdev {
@@ -273,8 +275,10 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
         if (threadIdx.x == 0) deviation[blockIdx.x] = make_float2(sa[0], sb[0]);
         if (!live) return;
     }
+    const float pressure = eos_pressure(k, density);
     reinterpret_cast<float*>(pos + t)[3] = density;
-    reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
+    reinterpret_cast<float*>(vel + t)[3] = pressure;
+    raise_range_guard(L.rangeGuard, !(density_in_range(density) && pressure_in_range(pressure)));
     if (nbrCount) {
         if (listed <= cap && flushed < listed) {
             // the last, partial group
@@ -360,10 +364,12 @@ __device__ __forceinline__ void density_finish(const StepConsts& k, float densit
                                                float4* __restrict__ vel, uint32_t t, uint32_t cap, uint32_t listed,
                                                uint32_t flushed, uint32_t (*ring)[kBlock],
                                                uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
-                                               size_t stride) {
+                                               size_t stride, uint32_t* rangeGuard) {
     density = density < 1e-6f ? 1e-6f : density;
+    const float pressure = eos_pressure(k, density);
     reinterpret_cast<float*>(pos + t)[3] = density;
-    reinterpret_cast<float*>(vel + t)[3] = eos_pressure(k, density);
+    reinterpret_cast<float*>(vel + t)[3] = pressure;
+    raise_range_guard(rangeGuard, !(density_in_range(density) && pressure_in_range(pressure)));
     if (nbrCount) {
         if (listed <= cap && flushed < listed) {
             const uint32_t r0 = flushed & (kRing - 1u), r = listed - flushed;
@@ -410,7 +416,7 @@ __device__ __forceinline__ void density_walk_global(const SceneDesc* d, uint32_t
                                                     float4* __restrict__ vel, const uint2* __restrict__ buckets,
                                                     uint4* __restrict__ nbrList,
                                                     uint32_t* __restrict__ nbrCount, uint32_t (*ring)[kBlock],
-                                                    size_t stride) {
+                                                    size_t stride, uint32_t* rangeGuard) {
     const StepConsts k = d->k;
     const uint32_t cellBase = d->cellOffset;
     const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
@@ -439,7 +445,7 @@ __device__ __forceinline__ void density_walk_global(const SceneDesc* d, uint32_t
             }
         }
     }
-    density_finish(k, density, pos, vel, t, cap, listed, flushed, ring, nbrList, nbrCount, stride);
+    density_finish(k, density, pos, vel, t, cap, listed, flushed, ring, nbrList, nbrCount, stride, rangeGuard);
 }
 
 __global__ void __launch_bounds__(kBlock, 8)
@@ -459,7 +465,7 @@ __global__ void __launch_bounds__(kBlock, 8)
             const uint2 r = rest[i];
             const SceneDesc* d = L.scenes + r.y;
             if (d->active)
-                density_walk_global(d, r.x, pos, vel, buckets, nbrList, nbrCount, ring, L.total);
+                density_walk_global(d, r.x, pos, vel, buckets, nbrList, nbrCount, ring, L.total, L.rangeGuard);
         }
         return;
     }
@@ -574,7 +580,7 @@ __global__ void __launch_bounds__(kBlock, 8)
                 if (keep) open((uint32_t)__builtin_ctz(keep));
             }
         }
-        density_finish(k, density, pos, vel, t, cap, listed, flushed, ring, nbrList, nbrCount, L.total);
+        density_finish(k, density, pos, vel, t, cap, listed, flushed, ring, nbrList, nbrCount, L.total, L.rangeGuard);
     }
 }
 
@@ -744,6 +750,11 @@ __device__ void field_one(const SphForceField& f, const float p[3], float out[3]
 
 // Pressure + viscosity contribution of neighbour slot u to particle (p, v, pressure),
 // accumulated into f in the reference's order (fluid_sph.cpp:244-254).
+// SHORT: the square root and the seven divisions by sqrt_rn / recip_rn / div_rn (sph_exact.h),
+// one reciprocal per divisor, for launches whose operands all lie in the ranges where those
+// give the IEEE results (the caller checks the scene's shortDivide and the range guard); the
+// results are the same bits, from ~100 instead of ~146 instructions per pair.
+template <bool SHORT>
 __device__ __forceinline__ void pair_force_from(const StepConsts& k, const float4 q, const float4 ov, float px,
                                                 float py, float pz, float vx, float vy, float vz, float pressure,
                                                 float f[3]) {
@@ -752,12 +763,27 @@ __device__ __forceinline__ void pair_force_from(const StepConsts& k, const float
     // r2 >= r2Cut <=> sqrtf(r2) >= h (both correctly rounded, monotone): the
     // reference's `distance >= h` test without the square root.
     if (r2 >= k.r2Cut) return;
-    const float distance = sqrtf(r2);
+    const float distance = SHORT ? sqrt_rn(r2) : sqrtf(r2);
     if (distance >= k.h || distance <= 1e-6f) return;
     const float otherDensity = q.w < 1e-6f ? 1e-6f : q.w;
     // spikyGradient (fluid_sph.cpp:33-39): ((c d) d) (delta / length)
     const float d = k.h - distance;
     const float g = (k.cSpiky * d) * d;
+    const float lap = k.cVisc * (k.h - distance);
+    if (SHORT) {
+        // a / (2 rho_j) is RN(a / rho_j) / 2: halving is exact for the normal quotients here
+        const float yd = recip_rn(distance), yr = recip_rn(otherDensity);
+        const float gx = g * div_rn(ddx, distance, yd), gy = g * div_rn(ddy, distance, yd),
+                    gz = g * div_rn(ddz, distance, yd);
+        const float sp = div_rn(k.negMass * (pressure + ov.w), otherDensity, yr) * 0.5f;
+        f[0] += sp * gx;
+        f[1] += sp * gy;
+        f[2] += sp * gz;
+        f[0] += div_rn(k.viscMass * (ov.x - vx), otherDensity, yr) * lap;
+        f[1] += div_rn(k.viscMass * (ov.y - vy), otherDensity, yr) * lap;
+        f[2] += div_rn(k.viscMass * (ov.z - vz), otherDensity, yr) * lap;
+        return;
+    }
     const float gx = g * (ddx / distance), gy = g * (ddy / distance), gz = g * (ddz / distance);
     // -mass (p_i + p_j) / (2 rho_j) * grad (fluid_sph.cpp:251-252)
     const float sp = (k.negMass * (pressure + ov.w)) / (2.0f * otherDensity);
@@ -765,7 +791,6 @@ __device__ __forceinline__ void pair_force_from(const StepConsts& k, const float
     f[1] += sp * gy;
     f[2] += sp * gz;
     // viscosity mass (v_j - v_i) / rho_j * laplacian (fluid_sph.cpp:253-254)
-    const float lap = k.cVisc * (k.h - distance);
     f[0] += ((k.viscMass * (ov.x - vx)) / otherDensity) * lap;
     f[1] += ((k.viscMass * (ov.y - vy)) / otherDensity) * lap;
     f[2] += ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
@@ -807,7 +832,7 @@ __device__ __forceinline__ void pair_force(const StepConsts& k, bool fast, const
     if (FAST && fast)
         pair_force_fast(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
     else
-        pair_force_from(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
+        pair_force_from<false>(k, pos[u], vel[u], px, py, pz, vx, vy, vz, pressure, f);
 }
 
 // 64 VGPRs at most (8 waves per SIMD): this VALU-bound pass runs ~4 % faster at 8 waves
@@ -858,24 +883,33 @@ __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
         }
     } else if (listed <= k.listCap) {
         // The in-range candidates recorded by the density pass, in visiting order.
-        for (uint32_t e = 0; e < listed; e += 4) {
-            const uint4 g = load_stream(nbrList + (size_t)(e >> 2) * L.total + t);
-            const uint32_t us[4] = {g.x, g.y, g.z, g.w};
-            // All four neighbours' state in flight before the first is consumed; slots
-            // past the end of the list re-read the last valid one and are not used.
-            float4 q[4], ov[4];
+        auto walk = [&](auto shortTag) {
+            constexpr bool SHORT = decltype(shortTag)::value;
+            for (uint32_t e = 0; e < listed; e += 4) {
+                const uint4 g = load_stream(nbrList + (size_t)(e >> 2) * L.total + t);
+                const uint32_t us[4] = {g.x, g.y, g.z, g.w};
+                // All four neighbours' state in flight before the first is consumed; slots
+                // past the end of the list re-read the last valid one and are not used.
+                float4 q[4], ov[4];
 #pragma unroll
-            for (uint32_t r = 0; r < 4; ++r) {
-                const uint32_t u = e + r < listed ? us[r] : us[0];
-                q[r] = pos[u];
-                ov[r] = vel[u];
-            }
+                for (uint32_t r = 0; r < 4; ++r) {
+                    const uint32_t u = e + r < listed ? us[r] : us[0];
+                    q[r] = pos[u];
+                    ov[r] = vel[u];
+                }
 #pragma unroll
-            for (uint32_t r = 0; r < 4; ++r) {
-                if (e + r < listed && us[r] != t)
-                    pair_force_from(k, q[r], ov[r], px, py, pz, vx, vy, vz, pressure, f);
+                for (uint32_t r = 0; r < 4; ++r) {
+                    if (e + r < listed && us[r] != t)
+                        pair_force_from<SHORT>(k, q[r], ov[r], px, py, pz, vx, vy, vz, pressure, f);
+                }
             }
-        }
+        };
+        // The short division when the scene's constants and every particle's state of this
+        // substep lie in its exact ranges (the same for the whole launch).
+        if (k.shortDivide && L.rangeGuard && *L.rangeGuard == 0u)
+            walk(std::true_type{});
+        else
+            walk(std::false_type{});
     } else {
         // No list (or it overflowed): walk the 27 buckets as the reference does.
         const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);

@@ -74,12 +74,15 @@ __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 // Bit mask of lanes at or below this lane.
 __device__ inline uint64_t lanes_upto(uint32_t lane) { return lane == 63u ? ~0ull : ((2ull << lane) - 1ull); }
 
-// buckets = (0, 0) and rep = ~0 over every scene's buckets, bin counts = 0 over every bin.
+// buckets = (0, 0) and rep = ~0 over every scene's buckets, bin counts = 0 over every bin;
+// the tiled pass's list and the range guard emptied.
 __global__ void __launch_bounds__(kBlock)
     k_grid_clear(uint2* __restrict__ buckets, uint32_t* __restrict__ rep, uint32_t cells,
-                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount) {
+                 uint32_t* __restrict__ binCount, uint32_t bins, uint32_t* __restrict__ restCount,
+                 uint32_t* __restrict__ rangeGuard) {
     const uint32_t stride = gridDim.x * kBlock;
     if (restCount && blockIdx.x == 0 && threadIdx.x == 0) *restCount = 0u;
+    if (rangeGuard && blockIdx.x == 0 && threadIdx.x == 0) *rangeGuard = 0u;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride) {
         buckets[i] = make_uint2(0u, 0u);
         rep[i] = 0xFFFFFFFFu;
@@ -320,14 +323,19 @@ __global__ void __launch_bounds__(kBlock)
     }
     const uint32_t dst = lo + rank;
     float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool outOfRange = false;
     if (w.valid) {
         p = pos[src];
+        const float4 v = vel[src];
         posSorted[dst] = p;
-        velSorted[dst] = vel[src];
+        velSorted[dst] = v;
         idsSorted[dst] = me.y;
         keysSorted[dst] = me.x;
         if (first) buckets[me.x].x = dst;
+        outOfRange = !(coord_in_range(p.x) && coord_in_range(p.y) && coord_in_range(p.z) &&
+                       velocity_in_range(v.x) && velocity_in_range(v.y) && velocity_in_range(v.z));
     }
+    raise_range_guard(L.rangeGuard, outOfRange);
     if (!rest) return;   // the same for every lane
     // Is the particle's own cell inside the half-tile its bin belongs to? (bins are cells when
     // shift == 0; the tiled pass is planned only then)
@@ -421,7 +429,8 @@ hipError_t launch_grid_clear(const Launch& L, const GridBuffers& b, hipStream_t 
     const uint32_t span = L.totalCells > L.totalBins ? L.totalCells : L.totalBins;
     uint32_t blocks = blocks_for(span);
     if (blocks > 4096u) blocks = 4096u;
-    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.buckets, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount);
+    k_grid_clear<<<blocks, kBlock, 0, s>>>(b.buckets, b.rep, L.totalCells, b.binStart, L.totalBins, b.restCount,
+                                           L.rangeGuard);
     return hipGetLastError();
 }
 

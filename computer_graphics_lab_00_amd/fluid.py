@@ -141,6 +141,13 @@ class FluidSolver:
         check(self._lib.sph_max_bin_occupancy(self._h, C.byref(out)))
         return int(out.value)
 
+    def short_division(self) -> bool:
+        """Whether the last substep's force pass used its short division and square root
+        (the same bits as the IEEE sequences, inside their exact operand ranges)."""
+        out = C.c_uint32()
+        check(self._lib.sph_short_division(self._h, C.byref(out)))
+        return bool(out.value)
+
     def velocities(self) -> np.ndarray:
         out = np.empty((self.particle_count(), 4), np.float32)
         check(self._lib.sph_read_particles(self._h, None, _fptr(out)))

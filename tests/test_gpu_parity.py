@@ -172,6 +172,36 @@ def test_tiled_density_dense_neighbourhoods(per_cell, oracle_lib):
     gpu.close()
 
 
+def test_short_division_path_and_its_fallbacks(oracle_lib):
+    """The force pass divides with its short sequences when the scene's constants and every
+    particle's state lie in their exact ranges, and with the IEEE ones otherwise; both are
+    bit-exact. The 16k dam break with random velocities takes the short path; the same state
+    with one velocity component of 1e-20 m/s, and a scene whose viscosity * mass is below
+    2^-40, take the IEEE path."""
+    say("case: short division and its fallbacks")
+    w = fluid.s16k()
+    p = w.params()
+    pos, vel = w.initial_state(p)
+    rng = np.random.default_rng(7)
+    vel[:, :3] = rng.normal(0, 0.5, (w.count, 3)).astype(np.float32)
+    dt = 1.0 / 600.0 / fluid.fluid_substeps(p, 1.0 / 600.0)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    assert gpu.short_division(), "the dam break should take the short path"
+    compare_substep(gpu, ora, p, "s16k short division")
+    gpu.close()
+    tiny = vel.copy()
+    tiny[1234, 0] = np.float32(1e-20)
+    gpu, ora = one_substep_both(p, pos, tiny, dt, oracle_lib)
+    assert not gpu.short_division(), "a velocity of 1e-20 is outside the short path's range"
+    compare_substep(gpu, ora, p, "s16k tiny velocity")
+    gpu.close()
+    p.viscosity = np.float32(1e-12)
+    gpu, ora = one_substep_both(p, pos, vel, dt, oracle_lib)
+    assert not gpu.short_division(), "viscosity * mass below 2^-40 keeps the IEEE path"
+    compare_substep(gpu, ora, p, "s16k tiny viscosity")
+    gpu.close()
+
+
 def test_neighbour_list_overflow_falls_back(oracle_lib):
     """cellCount = 1 and 40 particles: every particle has 27 x 32 in-range entries, far
     past the list capacity, so the force pass walks the buckets; still bit-exact."""
