@@ -311,13 +311,17 @@ def surrogate_leg(w, steps: int) -> dict | None:
     return out
 
 
-def profile_window(solver, p, pos_w, vel_w, steps: int) -> dict:
-    """Per-kernel event times of `steps` steps replayed eagerly from a saved state."""
+def profile_window(solver, p, pos_w, vel_w, steps: int, short: list | None = None) -> dict:
+    """Per-kernel event times of `steps` steps replayed eagerly from a saved state. With
+    `short`, appends for each step whether its last substep's force pass used the short
+    division (the same bits as the IEEE sequences)."""
     solver.load_state(pos_w, vel_w)
     solver.profile_enable(True)
     solver.profile_read()   # reset
     for _ in range(steps):
         solver.step(p, DT)
+        if short is not None:
+            short.append(solver.short_division())
     solver.synchronize()
     prof = solver.profile_read()
     solver.profile_enable(False)
@@ -413,7 +417,8 @@ def main():
     # launched eagerly with a HIP event pair around every kernel on the solver's stream
     # (events cannot sit inside the replayed graph). Kernel durations are the same either
     # way; launch gaps are not.
-    prof = profile_window(gpu, p, pos_w, vel_w, args.steps)
+    short = []
+    prof = profile_window(gpu, p, pos_w, vel_w, args.steps, short)
 
     value = n * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
@@ -448,6 +453,8 @@ def main():
         "kernels_valu_busy": {k: load_valu_busy(k) for k in prof},
         "profiled_steps": args.steps,
         "profiled_window": "the timed steps, replayed eagerly from the same saved state",
+        # steps of the window whose last substep's force pass divided with the short sequences
+        "force_short_division_steps": int(sum(short)),
     }
 
     # The same window in SPH_MODE_FAST_MATH (fused multiply-adds and hardware reciprocals,

@@ -2,7 +2,10 @@
 ``rocprofv3 --kernel-trace``): per-kernel launch statistics, and the busy time against the
 span of the last dispatches (the idle time between kernels, e.g. graph-replay launch gaps).
 
-    python -m computer_graphics_lab_00_amd.rocpd_summary RESULTS.db [--last N --steps K]
+    python -m computer_graphics_lab_00_amd.rocpd_summary RESULTS.db [--last N --steps K] [--range FIRST COUNT]
+
+--range restricts the statistics to launches FIRST .. FIRST + COUNT - 1 of each kernel (in
+time order), e.g. the window of steps a benchmark timed.
 """
 from __future__ import annotations
 
@@ -23,12 +26,15 @@ def _short(name: str) -> str:
     return key
 
 
-def kernel_stats(db: str) -> list[tuple[str, int, float, float, float, float]]:
-    """(kernel, launches, avg us, min us, max us, total ms), largest total first."""
-    rows = sqlite3.connect(db).execute("select name, start, end from kernels").fetchall()
+def kernel_stats(db: str, first: int = 0, count: int | None = None) -> list[tuple[str, int, float, float, float, float]]:
+    """(kernel, launches, avg us, min us, max us, total ms), largest total first; with `count`,
+    over launches first .. first + count - 1 of each kernel in time order."""
+    rows = sorted(sqlite3.connect(db).execute("select start, end, name from kernels").fetchall())
     agg: dict[str, list[float]] = {}
-    for name, s, e in rows:
+    for s, e, name in rows:
         agg.setdefault(_short(name), []).append((e - s) / 1000.0)
+    if count is not None:
+        agg = {k: v[first:first + count] for k, v in agg.items() if v[first:first + count]}
     out = [(k, len(v), sum(v) / len(v), min(v), max(v), sum(v) / 1000.0) for k, v in agg.items()]
     return sorted(out, key=lambda r: -r[5])
 
@@ -49,8 +55,10 @@ def main(argv=None) -> int:
     ap.add_argument("db")
     ap.add_argument("--last", type=int, default=0, help="dispatches at the end of the trace to check for gaps")
     ap.add_argument("--steps", type=int, default=1, help="steps those dispatches cover")
+    ap.add_argument("--range", type=int, nargs=2, metavar=("FIRST", "COUNT"), default=None,
+                    help="statistics over launches FIRST .. FIRST + COUNT - 1 of each kernel")
     a = ap.parse_args(argv)
-    stats = kernel_stats(a.db)
+    stats = kernel_stats(a.db, *(a.range or (0, None)))
     total = sum(r[5] for r in stats)
     print(f"{'kernel':40s} {'calls':>6s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'total_ms':>9s} {'pct':>6s}")
     for k, n, avg, mn, mx, tot in stats:

@@ -194,15 +194,15 @@ def test_params_save_load_roundtrip(sph, tmp_path):
     assert not np.any(a.params[SO.frozen_mask()])
 
 
-def _small_scene(seed=3, lattice=(16, 16, 8), warm=3):
+def _small_scene(seed=3, lattice=(16, 16, 8), warm=3, spacing=0.0049, flags=0):
     """A developed 2048-particle scene (h = 0.0098, 8 substeps per 1/600 s step)."""
-    wl = fluid.Workload("small", lattice, 0.0049, (0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 0.0098, cell_count=1 << 14,
+    wl = fluid.Workload("small", lattice, spacing, (0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 0.0098, cell_count=1 << 14,
                         corner=(0.05, 0.02, 0.06))
     p = wl.params()
     pos, _ = wl.initial_state(p)
     vel = np.zeros_like(pos)
     vel[:, :3] = surrogate.gaussian(seed, pos.shape[0] * 3, 0.5).reshape(-1, 3)
-    s = fluid.FluidSolver(pos.shape[0], 1 << 14, keep_forces=True)
+    s = fluid.FluidSolver(pos.shape[0], 1 << 14, keep_forces=True, flags=flags)
     s.load_state(pos, vel)
     for _ in range(warm):
         s.step(p, 1.0 / 600.0)
@@ -227,15 +227,23 @@ def _oracle_features(p, pos, vel, dt_sub, rho_check=None):
     return SO.bf16_bits(f), sums
 
 
-def test_features_match_the_oracle_bit_for_bit(sph, oracle_lib):
-    p, pos, vel, s = _small_scene()
+@pytest.mark.parametrize("spacing,flags", [(0.0049, 0), (0.0049, capi.CFG_BUCKET_WALK), (0.0028, 0)])
+def test_features_match_the_oracle_bit_for_bit(sph, oracle_lib, spacing, flags):
+    """The moment features (from the density pass's neighbourhood sums) against the oracle,
+    bit for bit: at spacing h / 2, in the bucket-walk configuration (no neighbour list kept),
+    and at spacing h / 3.5, where most particles have over 64 in-range candidates, so the
+    pass's ring of pending candidates fills and drains many times per particle."""
+    p, pos, vel, s = _small_scene(spacing=spacing, flags=flags)
     dt_sub = (1.0 / 600.0) / fluid.fluid_substeps(p, 1.0 / 600.0)
-    say(f"features n={pos.shape[0]} dt_sub={dt_sub}")
+    say(f"features n={pos.shape[0]} dt_sub={dt_sub} spacing={spacing} flags={flags}")
     x = surrogate.features(s, p, dt_sub)
     rho, o = _oracle_density(oracle_lib, p, pos, vel, dt_sub)
     want, sums = _oracle_features(p, pos, vel, dt_sub, rho_check=rho)
-    say(f"  in-range candidates: mean {sums[:, 3].mean():.1f}, min {sums[:, 3].min():.0f}")
+    say(f"  in-range candidates: mean {sums[:, 3].mean():.1f}, min {sums[:, 3].min():.0f}, "
+        f"over 64: {(sums[:, 3] > 64).mean():.2f}")
     assert sums[:, 3].mean() > 20
+    if spacing < 0.004:
+        assert (sums[:, 3] > 64).mean() > 0.5
     diff = x != want
     assert not diff.any(), f"{diff.sum()} feature values differ, first at {np.argwhere(diff)[:3].tolist()}"
     # the state is untouched by the feature pass

@@ -50,3 +50,9 @@ def test_busy_vs_span_counts_only_the_library_kernels(trace_db):
     assert g["span_ns"] == 4000 - 1100
     assert g["max_gap_ns"] == 100
     assert g["idle_fraction"] == pytest.approx(100 / 2900)
+
+
+def test_kernel_stats_over_a_launch_range(trace_db):
+    stats = {k: (n, avg) for k, n, avg, mn, mx, tot in R.kernel_stats(trace_db, 1, 5)}
+    # the second density launch only; kernels with a single launch drop out
+    assert stats == {"k_density<false,false,false>": (1, 2.0)}
