@@ -98,10 +98,10 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // pass reads a whole wave's group g in one coalesced access. A particle with more than
 // `listCap` entries stores count = listCap + 1 and the force pass scans its buckets
 // instead.
-// One lane per particle walks its 27 buckets. Candidates are read eight at a time from one
-// base address (a typical bucket is one batch, one wait for memory); the last batch of a
-// bucket may read up to seven slots past it (the position arrays carry kPosPad slots of
-// padding) and ignores them. Per candidate the work is the reference's arithmetic, a
+// One lane per particle walks its 27 buckets. Candidates are read eight at a time (four with
+// SUMS) from one base address (a typical bucket is one batch, one wait for memory); the last
+// batch of a bucket may read up to seven slots past it (the position arrays carry kPosPad
+// slots of padding) and ignores them. Per candidate the work is the reference's arithmetic, a
 // select and one bit of the batch's in-range mask. After the batch, its in-range slots
 // (~15 % of the candidates) go into the lane's ring of 16 pending entries in LDS, one
 // write each, and every completed group of four goes out to the list: no branch per
@@ -110,8 +110,8 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // stored it under a branch.)
 // With SUMS the pass also accumulates the neighbourhood sums of NeighbourSums
 // (sph_device.h) over the same candidates, in the same order, and builds no list. The
-// in-range slots go into a ring of 32 per lane in LDS instead, and whenever a lane of the
-// wave has more than 24 pending, every lane of the wave sums its pending candidates (their
+// in-range slots go into the lane's ring of 16 in LDS instead, and whenever a lane of the
+// wave has more than 12 pending, every lane of the wave sums its pending candidates (their
 // offsets recomputed from the positions with the visit's operations, so the sums are the
 // ones the visit would give). Summing inside the visit would run the ~45 instructions of
 // the sums for every candidate that any lane of the wave has in range; at 1M particles in a
@@ -120,14 +120,17 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // |rho / rho0 - 1| (the density check's first stage), in the tree the check's own reduction
 // uses, so the check needs no pass of its own.
 // DEDUPE = false is the variant for launches where no scene asked for de-duplicated
-// buckets: it keeps no bucket hashes live, which leaves the reference-mode pass at 64 VGPRs
-// (8 waves per SIMD, 4 VGPRs spilled, measured faster than one wave fewer) and the pass
-// with sums at 94 (5 waves, which its 32 KB of LDS per workgroup also allows).
+// buckets: it keeps no bucket hashes live. With the bucket descriptors fetched one dy row
+// (three buckets) at a time, the reference-mode pass fits 62 VGPRs and the pass with sums,
+// reading four candidates per batch, 64: 8 waves per SIMD for both (16 KB of LDS per
+// workgroup). Occupancy is what hides these passes' load latency: at 1M particles the pass
+// with sums ran 6 % faster rollouts this way than at 5 waves (nine descriptors per z layer,
+// eight candidates per batch, a ring of 32), and each wave fewer than 8 measured slower.
 // FAST (instantiated without SUMS only): scenes with SPH_MODE_FAST_MATH evaluate r2 with
 // fused multiply-adds and poly6 with the mass folded into its constant; other scenes of the
 // launch keep the reference's arithmetic.
 template <bool SUMS, bool DEDUPE, bool FAST = false>
-__global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) : 1)
+__global__ void __launch_bounds__(kBlock, !DEDUPE ? (FAST ? 6 : 8) : 1)
     k_density(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint2* __restrict__ buckets,
               uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount,
               float4* __restrict__ sums, float2* __restrict__ deviation) {
@@ -148,7 +151,11 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
     const uint32_t cap = (!SUMS && nbrList) ? k.listCap : 0u;
     float density = 0.0f;
     uint32_t listed = 0, flushed = 0;
-    __shared__ uint32_t ring[SUMS ? 32 : 16][kBlock];
+    // candidates per batch (SUMS: four, which keeps the pass at 8 waves per SIMD), and the
+    // ring of pending entries per lane: list entries, or with SUMS in-range slots to be summed
+    constexpr uint32_t kB = SUMS ? 4u : 8u;
+    constexpr uint32_t kSumRing = 16;
+    __shared__ uint32_t ring[16][kBlock];
     uint32_t pushed = 0, summed = 0;   // SUMS: in-range slots staged in the ring / summed
     // neighbourhood sums (SUMS only)
     float ax[20];
@@ -191,35 +198,38 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
     // positions with the visit's operations.
     auto sum_pending = [&]() {
         for (; summed < pushed; ++summed) {
-            const uint32_t u = ring[summed & 31u][threadIdx.x];
+            const uint32_t u = ring[summed & (kSumRing - 1u)][threadIdx.x];
             const float* q = reinterpret_cast<const float*>(pos + u);
             const float ddx = px - q[0], ddy = py - q[1], ddz = pz - q[2];
             const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
             accumulate_sums(ax, ddx, ddy, ddz, k.h2 - r2, vel[u], vx, vy, vz, invH2, invH);
         }
     };
-    for (int32_t dz = -1; dz <= 1; ++dz) {
-        // The nine buckets of one z layer: all descriptors are fetched before any is used.
-        uint32_t bk[9], cn[9], st[9];
+    for (int32_t dz = -1; dz <= 1; ++dz)
+    for (uint32_t g0 = 0; g0 < 9u; g0 += 3u) {
+        // The three buckets of one dy row: their descriptors are fetched before any is used.
+        uint32_t bk[3], cn[3], st[3];
 #pragma unroll
-        for (uint32_t o = 0; o < 9; ++o) {
-            bk[o] = hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz, k.cellMask);
-            const uint2 bd = buckets[cellBase + bk[o]];
-            st[o] = bd.x;
-            cn[o] = bd.y;
+        for (uint32_t oo = 0; oo < 3u; ++oo) {
+            const uint32_t o = g0 + oo;
+            bk[oo] = hash_cell(cx + (int32_t)(o % 3u) - 1, cy + (int32_t)(o / 3u) - 1, cz + dz, k.cellMask);
+            const uint2 bd = buckets[cellBase + bk[oo]];
+            st[oo] = bd.x;
+            cn[oo] = bd.y;
         }
 #pragma unroll
-        for (uint32_t o = 0; o < 9; ++o) {
-            if (dedupe && seen_before(cx, cy, cz, (uint32_t)(dz + 1) * 9u + o, bk[o], k.cellMask)) continue;
-            uint32_t cnt = live ? cn[o] : 0u;
+        for (uint32_t oo = 0; oo < 3u; ++oo) {
+            const uint32_t o = g0 + oo;
+            if (dedupe && seen_before(cx, cy, cz, (uint32_t)(dz + 1) * 9u + o, bk[oo], k.cellMask)) continue;
+            uint32_t cnt = live ? cn[oo] : 0u;
             if (cnt == 0) continue;
-            const uint32_t lo = st[o];
+            const uint32_t lo = st[oo];
             if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
             const float4* base = pos + lo;
-            for (uint32_t j = 0; j < cnt; j += 8) {
-                float qx[8], qy[8], qz[8];
+            for (uint32_t j = 0; j < cnt; j += kB) {
+                float qx[kB], qy[kB], qz[kB];
 #pragma unroll
-                for (uint32_t e = 0; e < 8; ++e) {
+                for (uint32_t e = 0; e < kB; ++e) {
                     const float* q = reinterpret_cast<const float*>(base + j + e);
                     qx[e] = q[0];
                     qy[e] = q[1];
@@ -227,14 +237,14 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (SUMS ? 5 : (FAST ? 6 : 8)) 
                 }
                 uint32_t m = 0;
 #pragma unroll
-                for (uint32_t e = 0; e < 8; ++e)
+                for (uint32_t e = 0; e < kB; ++e)
                     m |= visit(j + e < cnt, lo + j + e, qx[e], qy[e], qz[e]) ? (1u << e) : 0u;
                 if (SUMS) {
                     for (uint32_t mm = m; mm != 0u; mm &= mm - 1u) {
-                        ring[pushed & 31u][threadIdx.x] = lo + j + (uint32_t)__builtin_ctz(mm);
+                        ring[pushed & (kSumRing - 1u)][threadIdx.x] = lo + j + (uint32_t)__builtin_ctz(mm);
                         ++pushed;
                     }
-                    if (__any(pushed - summed > 24u)) sum_pending();   // every lane here empties its ring
+                    if (__any(pushed - summed > kSumRing - kB)) sum_pending();   // every lane here empties its ring
                 }
                 // The batch's in-range slots join the list in order, one per set bit, through
                 // this lane's ring of 16 pending entries in LDS; completed groups of four go out.
