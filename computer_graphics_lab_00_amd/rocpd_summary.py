@@ -2,10 +2,12 @@
 ``rocprofv3 --kernel-trace``): per-kernel launch statistics, and the busy time against the
 span of the last dispatches (the idle time between kernels, e.g. graph-replay launch gaps).
 
-    python -m computer_graphics_lab_00_amd.rocpd_summary RESULTS.db [--last N --steps K] [--range FIRST COUNT]
+    python -m computer_graphics_lab_00_amd.rocpd_summary RESULTS.db [--last N --steps K]
+        [--range FIRST COUNT | --tail COUNT]
 
 --range restricts the statistics to launches FIRST .. FIRST + COUNT - 1 of each kernel (in
-time order), e.g. the window of steps a benchmark timed.
+time order), e.g. the window of steps a benchmark timed; --tail to the last COUNT launches of
+each kernel.
 """
 from __future__ import annotations
 
@@ -26,15 +28,19 @@ def _short(name: str) -> str:
     return key
 
 
-def kernel_stats(db: str, first: int = 0, count: int | None = None) -> list[tuple[str, int, float, float, float, float]]:
+def kernel_stats(db: str, first: int = 0, count: int | None = None,
+                 tail: int | None = None) -> list[tuple[str, int, float, float, float, float]]:
     """(kernel, launches, avg us, min us, max us, total ms), largest total first; with `count`,
-    over launches first .. first + count - 1 of each kernel in time order."""
+    over launches first .. first + count - 1 of each kernel in time order; with `tail`, over
+    the last `tail` launches of each kernel."""
     rows = sorted(sqlite3.connect(db).execute("select start, end, name from kernels").fetchall())
     agg: dict[str, list[float]] = {}
     for s, e, name in rows:
         agg.setdefault(_short(name), []).append((e - s) / 1000.0)
     if count is not None:
         agg = {k: v[first:first + count] for k, v in agg.items() if v[first:first + count]}
+    if tail is not None:
+        agg = {k: v[-tail:] for k, v in agg.items()} if tail > 0 else {}
     out = [(k, len(v), sum(v) / len(v), min(v), max(v), sum(v) / 1000.0) for k, v in agg.items()]
     return sorted(out, key=lambda r: -r[5])
 
@@ -55,10 +61,13 @@ def main(argv=None) -> int:
     ap.add_argument("db")
     ap.add_argument("--last", type=int, default=0, help="dispatches at the end of the trace to check for gaps")
     ap.add_argument("--steps", type=int, default=1, help="steps those dispatches cover")
-    ap.add_argument("--range", type=int, nargs=2, metavar=("FIRST", "COUNT"), default=None,
-                    help="statistics over launches FIRST .. FIRST + COUNT - 1 of each kernel")
+    window = ap.add_mutually_exclusive_group()
+    window.add_argument("--range", type=int, nargs=2, metavar=("FIRST", "COUNT"), default=None,
+                        help="statistics over launches FIRST .. FIRST + COUNT - 1 of each kernel")
+    window.add_argument("--tail", type=int, default=None, metavar="COUNT",
+                        help="statistics over the last COUNT launches of each kernel")
     a = ap.parse_args(argv)
-    stats = kernel_stats(a.db, *(a.range or (0, None)))
+    stats = kernel_stats(a.db, *(a.range or (0, None)), tail=a.tail)
     total = sum(r[5] for r in stats)
     print(f"{'kernel':40s} {'calls':>6s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'total_ms':>9s} {'pct':>6s}")
     for k, n, avg, mn, mx, tot in stats:

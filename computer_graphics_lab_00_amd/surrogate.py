@@ -643,6 +643,34 @@ def compare(workload: Workload, net: SurrogateNet, steps: int, dt: float = 1.0 /
     return result
 
 
+def time_rollout(workload: Workload, net: SurrogateNet, steps: int, plain_steps: int = 5, warm_steps: int = 5,
+                 dt: float = 1.0 / 600.0, threshold: float = DEFAULT_THRESHOLD) -> dict:
+    """Wall time of `steps` surrogate steps of `workload` taken after `plain_steps` plain steps
+    and `warm_steps` surrogate steps from its initial state (a window of the rollout further
+    into the flow, for traces of the steady macro-step), with the rollout's statistics."""
+    p = workload.params()
+    pos0, vel0 = workload.initial_state(p)
+    solver = FluidSolver(pos0.shape[0], workload.cell_count)
+    try:
+        solver.load_state(pos0, vel0)
+        for _ in range(plain_steps):
+            solver.step(p, dt)
+        net.set_backoff(net.max_backoff)
+        if warm_steps:
+            rollout(solver, net, p, dt, warm_steps, threshold, SphRolloutStats())
+        solver.synchronize()
+        st = SphRolloutStats()
+        t0 = time.perf_counter()
+        rollout(solver, net, p, dt, steps, threshold, st)
+        solver.synchronize()
+        t = time.perf_counter() - t0
+    finally:
+        solver.close()
+    return {"particles": int(pos0.shape[0]), "first_step": plain_steps + warm_steps + 1, "steps": steps,
+            "ms_per_step": 1e3 * t / steps, "macro_steps": int(st.macroSteps), "fallbacks": int(st.fallbacks),
+            "backed_off": int(st.backedOff)}
+
+
 def main(argv=None) -> int:
     import argparse
     import json
@@ -680,6 +708,14 @@ def main(argv=None) -> int:
     e.add_argument("--error-every", type=int, default=5,
                    help="per-particle error against plain from the same state every this many steps")
     e.add_argument("--progress-every", type=int, default=20)
+    r = sp.add_parser("time", help="wall time of a window of the surrogate rollout further into a dam break")
+    r.add_argument("--weights", default=str(default_weights()))
+    r.add_argument("--workload", choices=("m1", "r4m", "s16k"), default="m1")
+    r.add_argument("--steps", type=int, default=10)
+    r.add_argument("--plain-steps", type=int, default=5, help="plain steps before the window")
+    r.add_argument("--warm-steps", type=int, default=5, help="surrogate steps before the window")
+    r.add_argument("--smoothing", type=float, default=DEFAULT_SMOOTHING)
+    r.add_argument("--backoff", type=int, default=DEFAULT_BACKOFF)
     v = sp.add_parser("validate", help="checked rollouts of held-out dam columns of the training family")
     v.add_argument("--weights", default=str(default_weights()))
     v.add_argument("--scenes", type=int, nargs="*", default=list(VALIDATION_SCENES))
@@ -724,6 +760,9 @@ def main(argv=None) -> int:
     net.set_smoothing(a.smoothing)
     net.set_backoff(a.backoff)
     wl = {"m1": fluid.m1, "r4m": fluid.r4m, "s16k": fluid.s16k}[a.workload]()
+    if a.cmd == "time":
+        print(json.dumps(time_rollout(wl, net, a.steps, a.plain_steps, a.warm_steps)), flush=True)
+        return 0
     res = compare(wl, net, a.steps, threshold=a.threshold, stats_every=a.stats_every, unchecked=not a.no_unchecked,
                   log=sys.stderr, progress_every=a.progress_every, error_every=a.error_every)
     if a.json:

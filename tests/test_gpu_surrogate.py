@@ -386,6 +386,19 @@ def test_rollout_split_over_calls_equals_one_call(sph):
     assert not np.array_equal(pa[:, :3], pos[:, :3])
 
 
+def test_time_rollout_window_equals_its_steps_taken_one_by_one(sph):
+    """surrogate.time_rollout (the `time` command) times the window after its plain and
+    surrogate warm-up steps; the state it times is the one the same calls make by hand."""
+    say("time_rollout window on the 16k dam break")
+    w = fluid.s16k()
+    net = _trained_net()
+    res = surrogate.time_rollout(w, net, steps=2, plain_steps=1, warm_steps=1)
+    assert res["particles"] == w.count and res["first_step"] == 3 and res["steps"] == 2
+    assert res["ms_per_step"] > 0.0
+    # 5 substeps per step at 16k: one macro-step per step, the fifth substep plain
+    assert res["macro_steps"] + res["backed_off"] == 2
+
+
 def test_rollout_is_never_served_a_stale_prediction(sph):
     p, pos, vel, a = _small_scene(seed=8)
     _, pos2, vel2, _ = _small_scene(seed=9)

@@ -56,3 +56,12 @@ def test_kernel_stats_over_a_launch_range(trace_db):
     stats = {k: (n, avg) for k, n, avg, mn, mx, tot in R.kernel_stats(trace_db, 1, 5)}
     # the second density launch only; kernels with a single launch drop out
     assert stats == {"k_density<false,false,false>": (1, 2.0)}
+
+
+def test_kernel_stats_over_the_last_launches(trace_db):
+    stats = {k: (n, avg) for k, n, avg, mn, mx, tot in R.kernel_stats(trace_db, tail=1)}
+    # the last launch of each kernel: the second density launch, the others' only one
+    assert stats["k_density<false,false,false>"] == (1, 2.0)
+    assert stats["k_force<false>"] == (1, 0.5)
+    assert stats["k_density<true,false,false>"] == (1, 0.3)
+    assert R.kernel_stats(trace_db, tail=0) == []
