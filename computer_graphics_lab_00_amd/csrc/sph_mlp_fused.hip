@@ -97,7 +97,12 @@ __device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias
     }
 }
 
-// One layer over the first KB k-blocks of its (LD-strided) staged weights.
+// One layer over the first KB k-blocks of its (LD-strided) staged weights. The four
+// 32-feature output tiles are finished two at a time (their MFMAs over every k-block, then
+// their epilogue), so the epilogue of one pair can issue while the next pair's MFMAs run,
+// and only four accumulators are live. Each accumulator still sums its k-blocks in order,
+// so the results are those of finishing all four tiles together; at 1M rows the network
+// pass ran 7 % faster this way (and 6 % one tile at a time).
 template <int KB, uint32_t LD, int KIN>
 __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                              uint32_t lane, const bf16x8 (&in)[2][KIN], bf16x8 (&out)[2][8],
@@ -105,30 +110,36 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ w, con
                                              uint32_t rows = 0) {
     const uint32_t r = lane & 31u, half = lane >> 5;
     const uint32_t ld = LD;
-    f32x16 acc[2][4];
+    constexpr int kPair = 2;
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
+    for (int m0 = 0; m0 < 4; m0 += kPair) {
+        f32x16 acc[2][kPair];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = f32x16{};
+        for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
+            for (int mm = 0; mm < kPair; ++mm) acc[nt][mm] = f32x16{};
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            const bf16x8 a = lds8(w + (uint32_t)(mt * 32 + r) * ld + (uint32_t)kb * 16u + 8u * half);
+        for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
-                acc[nt][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, in[nt][kb], acc[nt][mt], 0, 0, 0);
+            for (int mm = 0; mm < kPair; ++mm) {
+                const bf16x8 a = lds8(w + (uint32_t)((m0 + mm) * 32 + r) * ld + (uint32_t)kb * 16u + 8u * half);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+                    acc[nt][mm] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, in[nt][kb], acc[nt][mm], 0, 0, 0);
+            }
         }
-    }
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-        // with hOut: the activations of rows row0 .. row0 + 63, 128 BF16 per row
-        const uint32_t row = row0 + (uint32_t)nt * 32u + r;
-        uint16_t* dst = (hOut && row < rows) ? hOut + (size_t)row * 128u : nullptr;
+        for (int nt = 0; nt < 2; ++nt) {
+            // with hOut: the activations of rows row0 .. row0 + 63, 128 BF16 per row
+            const uint32_t row = row0 + (uint32_t)nt * 32u + r;
+            uint16_t* dst = (hOut && row < rows) ? hOut + (size_t)row * 128u : nullptr;
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-            to_operands(acc[nt][mt], bias + mt * 32, half, out[nt][2 * mt], out[nt][2 * mt + 1],
-                        dst ? dst + mt * 32 : nullptr);
+            for (int mm = 0; mm < kPair; ++mm) {
+                const int mt = m0 + mm;
+                to_operands(acc[nt][mm], bias + mt * 32, half, out[nt][2 * mt], out[nt][2 * mt + 1],
+                            dst ? dst + mt * 32 : nullptr);
+            }
+        }
     }
 }
 
