@@ -507,11 +507,11 @@ __global__ void __launch_bounds__(kBlock, 8)
     const uint32_t BX = (uint32_t)dim[0] + 2u, BY = (uint32_t)dim[1] + 2u, BZ = (uint32_t)dim[2] + 2u;
     const uint32_t BXY = BX * BY, nbox = BXY * BZ;   // <= kBoxCells: the planner checks the tile shape
 
-    // One thread per neighbourhood cell: its bucket's first slot, member count and the
-    // bounding box of the members the walk visits (read from the sorted state, which also
-    // brings them into the caches the lanes read them from next).
-    const uint32_t kc = threadIdx.x;
-    if (kc < nbox) {
+    // Per neighbourhood cell, one thread (striding, so every entry is written whatever the
+    // workgroup size): its bucket's first slot, member count and the bounding box of the
+    // members the walk visits (read from the sorted state, which also brings them into the
+    // caches the lanes read them from next).
+    for (uint32_t kc = threadIdx.x; kc < nbox; kc += kBlock) {
         const int32_t cx = bx0 + (int32_t)(kc % BX), cy = by0 + (int32_t)((kc / BX) % BY), cz = bz0 + (int32_t)(kc / BXY);
         const uint32_t b = cellBase + hash_cell(cx, cy, cz, k.cellMask);
         const uint2 bd = buckets[b];
