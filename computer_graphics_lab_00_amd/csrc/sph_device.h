@@ -30,7 +30,7 @@ constexpr uint32_t kScanPerThread = 8;    // elements each scan thread owns
 constexpr uint32_t kScanTile = kBlock * kScanPerThread;
 constexpr uint32_t kXcds = 8;             // MI355X: 8 XCDs, each with its own L2
 constexpr uint32_t kPosPad = 8;           // slots of padding after each position array: the
-                                          // density pass reads candidates eight at a time
+                                          // density pass reads candidates nine at a time
 constexpr uint32_t kHalfTileBins = 32;    // spatial bins per workgroup of the tiled density pass
 
 // Scalars of one substep of one scene, precomputed on the host exactly as the reference

@@ -98,10 +98,13 @@ __device__ __forceinline__ void accumulate_sums(float ax[20], float ddx, float d
 // pass reads a whole wave's group g in one coalesced access. A particle with more than
 // `listCap` entries stores count = listCap + 1 and the force pass scans its buckets
 // instead.
-// One lane per particle walks its 27 buckets. Candidates are read eight at a time (four with
+// One lane per particle walks its 27 buckets. Candidates are read nine at a time (four with
 // SUMS) from one base address (a typical bucket is one batch, one wait for memory); the last
-// batch of a bucket may read up to seven slots past it (the position arrays carry kPosPad
-// slots of padding) and ignores them. Per candidate the work is the reference's arithmetic, a
+// batch of a bucket may read up to eight slots past it (the position arrays carry kPosPad
+// slots of padding) and ignores them. Nine rather than eight because buckets of the dam
+// breaks' lattices (two particles per cell edge, jittered) hold 7 to 10 members: at 1M
+// particles the pass took 259 us with batches of nine against 266 us with eight (6: 261,
+// 7: 277, 10: 260, 12: 288), and the 64-scene batch 7.59 against 7.83 ms per step. Per candidate the work is the reference's arithmetic, a
 // select and one bit of the batch's in-range mask. After the batch, its in-range slots
 // (~15 % of the candidates) go into the lane's ring of 16 pending entries in LDS, one
 // write each, and every completed group of four goes out to the list: no branch per
@@ -153,8 +156,10 @@ __global__ void __launch_bounds__(kBlock, !DEDUPE ? (FAST ? 6 : 8) : 1)
     uint32_t listed = 0, flushed = 0;
     // candidates per batch (SUMS: four, which keeps the pass at 8 waves per SIMD), and the
     // ring of pending entries per lane: list entries, or with SUMS in-range slots to be summed
-    constexpr uint32_t kB = SUMS ? 4u : 8u;
+    constexpr uint32_t kB = SUMS ? 4u : 9u;
     constexpr uint32_t kSumRing = 16;
+    static_assert(kB - 1u <= kPosPad, "a batch reads at most kB - 1 slots past a bucket, into the padding");
+    static_assert(kB + 3u <= 16u, "the list ring holds a batch and the up to 3 entries pending before it");
     __shared__ uint32_t ring[16][kBlock];
     uint32_t pushed = 0, summed = 0;   // SUMS: in-range slots staged in the ring / summed
     // neighbourhood sums (SUMS only)
