@@ -172,6 +172,38 @@ def test_tiled_density_dense_neighbourhoods(per_cell, oracle_lib):
     gpu.close()
 
 
+@pytest.mark.parametrize("flags", [0, capi.CFG_BUCKET_WALK, capi.CFG_TILED_DENSITY])
+def test_bucket_sizes_across_batch_boundaries(flags, oracle_lib):
+    """Buckets of every member count from 1 to 33: two rows of cells along x, the counts
+    rising along one and falling along the other, so neighbourhoods mix them. The density
+    pass reads candidates nine at a time, so its batches end inside and at the edge of buckets
+    of 8, 9, 10, 17, 18, 19, 27 and 28 members, and at the 32-slot cap (33). One substep is
+    bit-exact, with the neighbour list, the bucket walk and the tiled pass."""
+    say(f"case: bucket sizes 1..33, flags {flags}")
+    h = 0.04
+    w = fluid.Workload("bucket_sizes", (33, 34, 1), h / 2, (0.0, 0.0, 0.0), (2.0, 1.0, 1.0), h,
+                       cell_count=32768)
+    p = w.params()
+    cells = []
+    for row, counts in ((10, range(1, 34)), (11, range(33, 0, -1))):
+        for i, k in enumerate(counts):
+            cells += [(2 + i, row, 10)] * k
+    assert len(cells) == w.count
+    rng = np.random.default_rng(11)
+    c = np.array(cells, np.float64)
+    pos = np.zeros((len(cells), 4), np.float32)
+    pos[:, :3] = ((c + rng.uniform(0.05, 0.95, c.shape)) * h).astype(np.float32)
+    pos[:, 3] = np.float32(p.restDensity)
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(0, 0.5, (len(cells), 3)).astype(np.float32)
+    gpu, ora = one_substep_both(p, pos, vel, 1.0 / 3000.0, oracle_lib, flags=flags)
+    _, counts, _ = gpu.grid(p.cellCount)
+    sizes = set(int(v) for v in counts)
+    assert {8, 9, 10, 17, 18, 19, 27, 28} <= sizes and max(sizes) >= 32, "the bucket sizes are present"
+    compare_substep(gpu, ora, p, f"bucket sizes 1..33, flags {flags}")
+    gpu.close()
+
+
 def test_short_division_path_and_its_fallbacks(oracle_lib):
     """The force pass divides with its short sequences when the scene's constants and every
     particle's state lie in their exact ranges, and with the IEEE ones otherwise; both are
