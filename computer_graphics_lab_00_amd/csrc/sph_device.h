@@ -245,6 +245,10 @@ struct DensityTiles {
     uint32_t restBlocks;
 };
 
+// Single-scene launches of at most this many particle slots (capped buckets, the reference's
+// arithmetic, neighbour list kept) take the small-scene density pass: one wave per particle.
+constexpr uint32_t kSmallDensityMax = 32768;
+
 // launchers (sph_forces.hip)
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2* buckets, uint4* nbrList,
                           uint32_t* nbrCount,

@@ -999,6 +999,110 @@ __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
     velOut[t] = make_float4(vv[0], vv[1], vv[2], pressure);
 }
 
+// ---- small-scene density pass ----
+//
+// A scene of a few thousand particles gives the lockstep pass one wave per CU or fewer, and
+// each lane's walk over its 27 buckets is then pure memory latency (16,384 particles: 46 us
+// for 64 workgroups). Here one wave takes one particle: lane o < 27 walks neighbour bucket o
+// (its members in order, with the lockstep pass's operations) and stages its in-range terms
+// and slots in LDS; then lane 0 adds the terms bucket by bucket in the reference's visiting
+// order (out-of-range candidates, which add +0 there, are simply absent: the sum is the same
+// bits) and applies the EOS, and every lane writes its slots to their places in the
+// neighbour list.
+constexpr uint32_t kSmallWave = 64;   // threads per workgroup: one particle
+
+__global__ void __launch_bounds__(kSmallWave)
+    k_density_small(Launch L, float4* __restrict__ pos, float4* __restrict__ vel, const uint2* __restrict__ buckets,
+                    uint4* __restrict__ nbrList, uint32_t* __restrict__ nbrCount) {
+    __shared__ float sTerm[27 * SPH_CELL_CAPACITY];
+    __shared__ uint32_t sSlot[27 * SPH_CELL_CAPACITY];
+    const SceneDesc* d = L.scenes;   // single-scene launches only
+    const uint32_t local = blockIdx.x;
+    if (!d->active || local >= d->k.n) return;   // the same for the whole workgroup
+    const StepConsts& k = d->k;
+    const uint32_t t = d->offset + local, o = threadIdx.x;
+    const float* self = reinterpret_cast<const float*>(pos + t);
+    const float px = self[0], py = self[1], pz = self[2];
+    uint32_t m = 0;
+    if (o < 27u) {
+        const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
+        // visiting order: dz outer, then dy, dx inner (fluid_sph.cpp:205-216)
+        const int32_t dx = (int32_t)(o % 3u) - 1, dy = (int32_t)((o / 3u) % 3u) - 1, dz = (int32_t)(o / 9u) - 1;
+        const uint2 bd = buckets[d->cellOffset + hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask)];
+        const uint32_t cnt = bd.y < SPH_CELL_CAPACITY ? bd.y : SPH_CELL_CAPACITY;
+        // eight candidates per batch from one base address (the last batch may read up to
+        // seven slots past the bucket, into the next bucket or the padding, and ignores them)
+        static_assert(7u <= kPosPad, "reads past a bucket stay inside the padding");
+        for (uint32_t j = 0; j < cnt; j += 8u) {
+            float qx[8], qy[8], qz[8];
+#pragma unroll
+            for (uint32_t e = 0; e < 8u; ++e) {
+                const float* q = reinterpret_cast<const float*>(pos + bd.x + j + e);
+                qx[e] = q[0];
+                qy[e] = q[1];
+                qz[e] = q[2];
+            }
+#pragma unroll
+            for (uint32_t e = 0; e < 8u; ++e) {
+                const float ddx = px - qx[e], ddy = py - qy[e], ddz = pz - qz[e];
+                const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                if (j + e < cnt && r2 < k.h2) {
+                    const float dd = k.h2 - r2;
+                    sTerm[o * SPH_CELL_CAPACITY + m] = k.mass * (((k.cPoly6 * dd) * dd) * dd);
+                    sSlot[o * SPH_CELL_CAPACITY + m] = bd.x + j + e;
+                    ++m;
+                }
+            }
+        }
+    }
+    // each bucket's place in the particle's in-range sequence: exclusive scan of m over lanes
+    uint32_t before = m;
+#pragma unroll
+    for (uint32_t w = 1; w < 32u; w <<= 1) {
+        const uint32_t up = __shfl_up(before, w, 64);
+        if (o >= w) before += up;
+    }
+    before -= m;
+    const uint32_t listed = __shfl(before + m, 26, 64);
+    __syncthreads();
+    const uint32_t cap = nbrList ? k.listCap : 0u;
+    if (nbrCount && listed <= cap) {
+        // this bucket's slots, then zeros after the last entry up to its group's end
+        uint32_t* list = reinterpret_cast<uint32_t*>(nbrList);
+        for (uint32_t e = 0; e < m; ++e) {
+            const uint32_t at = before + e;
+            list[((size_t)(at >> 2) * L.total + t) * 4u + (at & 3u)] = sSlot[o * SPH_CELL_CAPACITY + e];
+        }
+        if (o < ((4u - (listed & 3u)) & 3u)) {
+            const uint32_t at = listed + o;
+            list[((size_t)(at >> 2) * L.total + t) * 4u + (at & 3u)] = 0u;
+        }
+    } else if (nbrCount && cap) {
+        // past the cap: the complete groups of four within the first cap entries are kept, as
+        // the lockstep pass keeps them (the force pass then walks the buckets itself)
+        uint32_t* list = reinterpret_cast<uint32_t*>(nbrList);
+        for (uint32_t e = 0; e < m && before + e < (cap & ~3u); ++e) {
+            const uint32_t at = before + e;
+            list[((size_t)(at >> 2) * L.total + t) * 4u + (at & 3u)] = sSlot[o * SPH_CELL_CAPACITY + e];
+        }
+    }
+    float density = 0.0f;
+    if (o == 0u) {
+        for (uint32_t b = 0; b < 27u; ++b) {
+            const uint32_t c = __builtin_amdgcn_readlane(m, b);
+            for (uint32_t e = 0; e < c; ++e) density = density + sTerm[b * SPH_CELL_CAPACITY + e];
+        }
+    }
+    density = density < 1e-6f ? 1e-6f : density;
+    const float pressure = eos_pressure(k, density);
+    raise_range_guard(L.rangeGuard, o == 0u && !(density_in_range(density) && pressure_in_range(pressure)));
+    if (o == 0u) {
+        reinterpret_cast<float*>(pos + t)[3] = density;
+        reinterpret_cast<float*>(vel + t)[3] = pressure;
+        if (nbrCount) nbrCount[t] = cap && listed <= cap ? listed : cap + 1u;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2* buckets, uint4* nbrList,
@@ -1006,6 +1110,13 @@ hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2
                           const NeighbourSums* sums, hipStream_t s, const DensityTiles* tiles) {
     if (L.blocks == 0) return hipSuccess;
     const bool dedupe = (L.modes & SPH_MODE_DEDUPE) != 0;
+    const uint32_t plainModes = SPH_MODE_DEDUPE | SPH_MODE_UNCAPPED | SPH_MODE_FAST_MATH;
+    if (!sums && !tiles && nbrList && nbrCount && L.numScenes == 1 && !(L.modes & plainModes) &&
+        L.blocks * kBlock <= kSmallDensityMax) {
+        // one workgroup per particle slot of the scene's workgroups (the kernel skips those past n)
+        k_density_small<<<L.blocks * kBlock, kSmallWave, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount);
+        return hipGetLastError();
+    }
     if (sums) {
         if (dedupe)
             k_density<true, true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nullptr, nullptr,
