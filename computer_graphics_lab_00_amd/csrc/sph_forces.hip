@@ -769,17 +769,20 @@ __device__ void field_one(const SphForceField& f, const float p[3], float out[3]
 // one reciprocal per divisor, for launches whose operands all lie in the ranges where those
 // give the IEEE results (the caller checks the scene's shortDivide and the range guard); the
 // results are the same bits, from ~100 instead of ~146 instructions per pair.
+// One pair's contribution to the force on particle (px, py, pz): the pressure term a and the
+// viscosity term b, each per component; false when the pair is out of range. The reference adds
+// them to the force as f += a, then f += b (pair_force_from).
 template <bool SHORT>
-__device__ __forceinline__ void pair_force_from(const StepConsts& k, const float4 q, const float4 ov, float px,
-                                                float py, float pz, float vx, float vy, float vz, float pressure,
-                                                float f[3]) {
+__device__ __forceinline__ bool pair_terms(const StepConsts& k, const float4 q, const float4 ov, float px, float py,
+                                           float pz, float vx, float vy, float vz, float pressure, float a[3],
+                                           float b[3]) {
     const float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
     const float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
     // r2 >= r2Cut <=> sqrtf(r2) >= h (both correctly rounded, monotone): the
     // reference's `distance >= h` test without the square root.
-    if (r2 >= k.r2Cut) return;
+    if (r2 >= k.r2Cut) return false;
     const float distance = SHORT ? sqrt_rn(r2) : sqrtf(r2);
-    if (distance >= k.h || distance <= 1e-6f) return;
+    if (distance >= k.h || distance <= 1e-6f) return false;
     const float otherDensity = q.w < 1e-6f ? 1e-6f : q.w;
     // spikyGradient (fluid_sph.cpp:33-39): ((c d) d) (delta / length)
     const float d = k.h - distance;
@@ -791,24 +794,39 @@ __device__ __forceinline__ void pair_force_from(const StepConsts& k, const float
         const float gx = g * div_rn(ddx, distance, yd), gy = g * div_rn(ddy, distance, yd),
                     gz = g * div_rn(ddz, distance, yd);
         const float sp = div_rn(k.negMass * (pressure + ov.w), otherDensity, yr) * 0.5f;
-        f[0] += sp * gx;
-        f[1] += sp * gy;
-        f[2] += sp * gz;
-        f[0] += div_rn(k.viscMass * (ov.x - vx), otherDensity, yr) * lap;
-        f[1] += div_rn(k.viscMass * (ov.y - vy), otherDensity, yr) * lap;
-        f[2] += div_rn(k.viscMass * (ov.z - vz), otherDensity, yr) * lap;
-        return;
+        a[0] = sp * gx;
+        a[1] = sp * gy;
+        a[2] = sp * gz;
+        b[0] = div_rn(k.viscMass * (ov.x - vx), otherDensity, yr) * lap;
+        b[1] = div_rn(k.viscMass * (ov.y - vy), otherDensity, yr) * lap;
+        b[2] = div_rn(k.viscMass * (ov.z - vz), otherDensity, yr) * lap;
+        return true;
     }
     const float gx = g * (ddx / distance), gy = g * (ddy / distance), gz = g * (ddz / distance);
     // -mass (p_i + p_j) / (2 rho_j) * grad (fluid_sph.cpp:251-252)
     const float sp = (k.negMass * (pressure + ov.w)) / (2.0f * otherDensity);
-    f[0] += sp * gx;
-    f[1] += sp * gy;
-    f[2] += sp * gz;
+    a[0] = sp * gx;
+    a[1] = sp * gy;
+    a[2] = sp * gz;
     // viscosity mass (v_j - v_i) / rho_j * laplacian (fluid_sph.cpp:253-254)
-    f[0] += ((k.viscMass * (ov.x - vx)) / otherDensity) * lap;
-    f[1] += ((k.viscMass * (ov.y - vy)) / otherDensity) * lap;
-    f[2] += ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
+    b[0] = ((k.viscMass * (ov.x - vx)) / otherDensity) * lap;
+    b[1] = ((k.viscMass * (ov.y - vy)) / otherDensity) * lap;
+    b[2] = ((k.viscMass * (ov.z - vz)) / otherDensity) * lap;
+    return true;
+}
+
+template <bool SHORT>
+__device__ __forceinline__ void pair_force_from(const StepConsts& k, const float4 q, const float4 ov, float px,
+                                                float py, float pz, float vx, float vy, float vz, float pressure,
+                                                float f[3]) {
+    float a[3], b[3];
+    if (!pair_terms<SHORT>(k, q, ov, px, py, pz, vx, vy, vz, pressure, a, b)) return;
+    f[0] += a[0];
+    f[1] += a[1];
+    f[2] += a[2];
+    f[0] += b[0];
+    f[1] += b[1];
+    f[2] += b[2];
 }
 
 // The same contribution for scenes with SPH_MODE_FAST_MATH: r2 by fused multiply-adds (the
@@ -852,6 +870,93 @@ __device__ __forceinline__ void pair_force(const StepConsts& k, bool fast, const
 
 // 64 VGPRs at most (8 waves per SIMD): this VALU-bound pass runs ~4 % faster at 8 waves
 // than at the 6 its natural register count allows, with no spills.
+// No list (or it overflowed): the force on particle t from its 27 buckets, walked as the
+// reference walks them (fluid_sph.cpp:234-258).
+template <bool FAST>
+__device__ __forceinline__ void bucket_walk_force(const StepConsts& k, bool fast, uint32_t cellBase,
+                                                  const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                  const uint2* __restrict__ buckets, uint32_t t, float px, float py,
+                                                  float pz, float vx, float vy, float vz, float pressure,
+                                                  float f[3]) {
+    const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
+    const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
+    const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
+    uint32_t o = 0;
+    for (int32_t dz = -1; dz <= 1; ++dz) {
+        for (int32_t dy = -1; dy <= 1; ++dy) {
+            for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
+                const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
+                if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
+                const uint2 bd = buckets[cellBase + b];
+                uint32_t cnt = bd.y;
+                if (cnt == 0) continue;
+                const uint32_t lo = bd.x;
+                if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const uint32_t u = lo + j;
+                    if (u == t) continue;
+                    pair_force<FAST>(k, fast, pos, vel, u, px, py, pz, vx, vy, vz, pressure, f);
+                }
+            }
+        }
+    }
+}
+
+// The integration of particle t from its pressure + viscosity force f: acceleration, symplectic
+// Euler, speed clamp, container walls and colliders (fluid_sph.cpp:260-300); writes its new
+// state (and, when asked, the force).
+__device__ __forceinline__ void integrate_write(const StepConsts& k, const BoundaryConsts& bnd, uint32_t t, float px,
+                                                float py, float pz, float vx, float vy, float vz, float density,
+                                                float pressure, const float f[3], float4* __restrict__ posOut,
+                                                float4* __restrict__ velOut, float4* __restrict__ forceOut) {
+    const float fx = f[0], fy = f[1], fz = f[2];
+    if (forceOut) forceOut[t] = make_float4(fx, fy, fz, 0.0f);
+
+    // acceleration = force / density + gravity + fields (fluid_sph.cpp:260-261)
+    float fa[3] = {0.0f, 0.0f, 0.0f};
+    const float here[3] = {px, py, pz};
+    for (uint32_t f = 0; f < k.fieldCount; ++f) {
+        float a[3];
+        field_one(bnd.fields[f], here, a);
+        fa[0] += a[0]; fa[1] += a[1]; fa[2] += a[2];
+    }
+    const float ax = (fx / density + k.gravity[0]) + fa[0];
+    const float ay = (fy / density + k.gravity[1]) + fa[1];
+    const float az = (fz / density + k.gravity[2]) + fa[2];
+    vx += ax * k.dt;
+    vy += ay * k.dt;
+    vz += az * k.dt;
+    // speed clamp (fluid_sph.cpp:264-268)
+    const float speed = sqrtf((vx * vx + vy * vy) + vz * vz);
+    if (speed > k.maxSpeed) {
+        const float sc = k.maxSpeed / speed;
+        vx *= sc;
+        vy *= sc;
+        vz *= sc;
+    }
+    px += vx * k.dt;
+    py += vy * k.dt;
+    pz += vz * k.dt;
+    // container walls (fluid_sph.cpp:271-285)
+    float pp[3] = {px, py, pz};
+    float vv[3] = {vx, vy, vz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (pp[a] < k.low[a]) {
+            pp[a] = k.low[a];
+            if (vv[a] < 0.0f) vv[a] = -vv[a] * k.restitution;
+        } else if (pp[a] > k.high[a]) {
+            pp[a] = k.high[a];
+            if (vv[a] > 0.0f) vv[a] = -vv[a] * k.restitution;
+        }
+    }
+    // colliders (fluid_sph.cpp:289-300)
+    for (uint32_t c = 0; c < k.colliderCount; ++c) collide(bnd.colliders[c], k.radius, k.restitution, pp, vv);
+
+    posOut[t] = make_float4(pp[0], pp[1], pp[2], density);
+    velOut[t] = make_float4(vv[0], vv[1], vv[2], pressure);
+}
+
 template <bool FAST>
 __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
     k_force(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -927,76 +1032,10 @@ __global__ void __launch_bounds__(kBlock, FAST ? 5 : 8)
             walk(std::false_type{});
     } else {
         // No list (or it overflowed): walk the 27 buckets as the reference does.
-        const int32_t cx = cell_coord(px, k.h), cy = cell_coord(py, k.h), cz = cell_coord(pz, k.h);
-        const bool capped = (k.mode & SPH_MODE_UNCAPPED) == 0;
-        const bool dedupe = (k.mode & SPH_MODE_DEDUPE) != 0;
-        uint32_t o = 0;
-        for (int32_t dz = -1; dz <= 1; ++dz) {
-            for (int32_t dy = -1; dy <= 1; ++dy) {
-                for (int32_t dx = -1; dx <= 1; ++dx, ++o) {
-                    const uint32_t b = hash_cell(cx + dx, cy + dy, cz + dz, k.cellMask);
-                    if (dedupe && seen_before(cx, cy, cz, o, b, k.cellMask)) continue;
-                    const uint2 bd = buckets[cellBase + b];
-                    uint32_t cnt = bd.y;
-                    if (cnt == 0) continue;
-                    const uint32_t lo = bd.x;
-                    if (capped && cnt > SPH_CELL_CAPACITY) cnt = SPH_CELL_CAPACITY;
-                    for (uint32_t j = 0; j < cnt; ++j) {
-                        const uint32_t u = lo + j;
-                        if (u == t) continue;
-                        pair_force<FAST>(k, fast, pos, vel, u, px, py, pz, vx, vy, vz, pressure, f);
-                    }
-                }
-            }
-        }
+        bucket_walk_force<FAST>(k, fast, cellBase, pos, vel, buckets, t, px, py, pz, vx, vy, vz, pressure, f);
     }
-    const float fx = f[0], fy = f[1], fz = f[2];
-    if (forceOut) forceOut[t] = make_float4(fx, fy, fz, 0.0f);
-
-    // acceleration = force / density + gravity + fields (fluid_sph.cpp:260-261)
-    float fa[3] = {0.0f, 0.0f, 0.0f};
-    const float here[3] = {px, py, pz};
     const BoundaryConsts& bnd = L.bounds[L.blockScene[xcd_block(blockIdx.x, gridDim.x)]];
-    for (uint32_t f = 0; f < k.fieldCount; ++f) {
-        float a[3];
-        field_one(bnd.fields[f], here, a);
-        fa[0] += a[0]; fa[1] += a[1]; fa[2] += a[2];
-    }
-    const float ax = (fx / density + k.gravity[0]) + fa[0];
-    const float ay = (fy / density + k.gravity[1]) + fa[1];
-    const float az = (fz / density + k.gravity[2]) + fa[2];
-    vx += ax * k.dt;
-    vy += ay * k.dt;
-    vz += az * k.dt;
-    // speed clamp (fluid_sph.cpp:264-268)
-    const float speed = sqrtf((vx * vx + vy * vy) + vz * vz);
-    if (speed > k.maxSpeed) {
-        const float sc = k.maxSpeed / speed;
-        vx *= sc;
-        vy *= sc;
-        vz *= sc;
-    }
-    px += vx * k.dt;
-    py += vy * k.dt;
-    pz += vz * k.dt;
-    // container walls (fluid_sph.cpp:271-285)
-    float pp[3] = {px, py, pz};
-    float vv[3] = {vx, vy, vz};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (pp[a] < k.low[a]) {
-            pp[a] = k.low[a];
-            if (vv[a] < 0.0f) vv[a] = -vv[a] * k.restitution;
-        } else if (pp[a] > k.high[a]) {
-            pp[a] = k.high[a];
-            if (vv[a] > 0.0f) vv[a] = -vv[a] * k.restitution;
-        }
-    }
-    // colliders (fluid_sph.cpp:289-300)
-    for (uint32_t c = 0; c < k.colliderCount; ++c) collide(bnd.colliders[c], k.radius, k.restitution, pp, vv);
-
-    posOut[t] = make_float4(pp[0], pp[1], pp[2], density);
-    velOut[t] = make_float4(vv[0], vv[1], vv[2], pressure);
+    integrate_write(k, bnd, t, px, py, pz, vx, vy, vz, density, pressure, f, posOut, velOut, forceOut);
 }
 
 // ---- small-scene density pass ----
@@ -1103,6 +1142,73 @@ __global__ void __launch_bounds__(kSmallWave)
     }
 }
 
+// The force pass for the same small single-scene launches: one wave per particle, lane e < 64
+// computing the contributions of list entry e, lane 0 adding them in list order (each pair's
+// pressure term, then its viscosity term, as pair_force_from adds them) and integrating.
+// Particles whose list overflowed walk their buckets on lane 0 as in k_force.
+__global__ void __launch_bounds__(kSmallWave)
+    k_force_small(Launch L, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                  const uint2* __restrict__ buckets, const uint4* __restrict__ nbrList,
+                  const uint32_t* __restrict__ nbrCount, float4* __restrict__ posOut, float4* __restrict__ velOut,
+                  float4* __restrict__ forceOut) {
+    static_assert(kSmallWave >= 64u, "one lane per list entry of the 64-entry list");
+    __shared__ float sA[64][3], sB[64][3];
+    __shared__ uint32_t sIn[64];
+    const SceneDesc* d = L.scenes;   // single-scene launches only
+    const uint32_t local = blockIdx.x;
+    if (local >= d->k.n) return;     // the same for the whole workgroup
+    const uint32_t t = d->offset + local, e = threadIdx.x;
+    if (!d->active) {
+        if (e == 0u) {
+            posOut[t] = pos[t];
+            velOut[t] = vel[t];
+        }
+        return;
+    }
+    const StepConsts& k = d->k;
+    const float4 self = pos[t];
+    const float4 selfVel = vel[t];
+    const float px = self.x, py = self.y, pz = self.z;
+    const float density = self.w < 1e-6f ? 1e-6f : self.w;
+    const float vx = selfVel.x, vy = selfVel.y, vz = selfVel.z;
+    const float pressure = selfVel.w;
+    const uint32_t listed = nbrCount[t];
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    if (listed <= k.listCap && listed <= 64u) {
+        const bool shortDiv = k.shortDivide && L.rangeGuard && *L.rangeGuard == 0u;
+        uint32_t in = 0u;
+        if (e < listed) {
+            const uint32_t u = reinterpret_cast<const uint32_t*>(nbrList)[((size_t)(e >> 2) * L.total + t) * 4u + (e & 3u)];
+            if (u != t) {
+                float a[3], b[3];
+                const float4 q = pos[u], ov = vel[u];
+                in = shortDiv ? pair_terms<true>(k, q, ov, px, py, pz, vx, vy, vz, pressure, a, b)
+                              : pair_terms<false>(k, q, ov, px, py, pz, vx, vy, vz, pressure, a, b);
+                if (in) {
+                    sA[e][0] = a[0]; sA[e][1] = a[1]; sA[e][2] = a[2];
+                    sB[e][0] = b[0]; sB[e][1] = b[1]; sB[e][2] = b[2];
+                }
+            }
+        }
+        sIn[e] = in;
+        __syncthreads();
+        if (e != 0u) return;
+        for (uint32_t r = 0; r < listed; ++r) {
+            if (!sIn[r]) continue;
+            f[0] += sA[r][0];
+            f[1] += sA[r][1];
+            f[2] += sA[r][2];
+            f[0] += sB[r][0];
+            f[1] += sB[r][1];
+            f[2] += sB[r][2];
+        }
+    } else {
+        if (e != 0u) return;
+        bucket_walk_force<false>(k, false, d->cellOffset, pos, vel, buckets, t, px, py, pz, vx, vy, vz, pressure, f);
+    }
+    integrate_write(k, L.bounds[0], t, px, py, pz, vx, vy, vz, density, pressure, f, posOut, velOut, forceOut);
+}
+
 }  // namespace
 
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2* buckets, uint4* nbrList,
@@ -1150,6 +1256,12 @@ hipError_t launch_force(const Launch& L, const float4* pos, const float4* vel, c
                         const uint4* nbrList, const uint32_t* nbrCount, float4* posOut,
                         float4* velOut, float4* forceOut, hipStream_t s) {
     if (L.blocks == 0) return hipSuccess;
+    if (nbrList && nbrCount && L.numScenes == 1 && !(L.modes & SPH_MODE_FAST_MATH) &&
+        L.blocks * kBlock <= kSmallDensityMax) {
+        k_force_small<<<L.blocks * kBlock, kSmallWave, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount, posOut,
+                                                               velOut, forceOut);
+        return hipGetLastError();
+    }
     if (L.modes & SPH_MODE_FAST_MATH)
         k_force<true><<<L.blocks, kBlock, 0, s>>>(L, pos, vel, buckets, nbrList, nbrCount, posOut, velOut,
                                                   forceOut);
