@@ -246,8 +246,10 @@ struct DensityTiles {
 };
 
 // Single-scene launches of at most this many particle slots (capped buckets, the reference's
-// arithmetic, neighbour list kept) take the small-scene density pass: one wave per particle.
-constexpr uint32_t kSmallDensityMax = 32768;
+// arithmetic, neighbour list kept) take the small-scene density and force passes: one wave
+// per particle. Measured on dam breaks (M particle-steps/s, small passes against lockstep):
+// 16k 51 against 38, 32k 68 against 75, 64k 83 against 145; the crossover lies near 27k.
+constexpr uint32_t kSmallDensityMax = 24576;
 
 // launchers (sph_forces.hip)
 hipError_t launch_density(const Launch& L, float4* pos, float4* vel, const uint2* buckets, uint4* nbrList,
